@@ -1,0 +1,262 @@
+"""ctypes bindings of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by the cpu_baseline leg of bench.py, always as the checker.  The product
+path (baum_amd) never imports this package.
+
+Arrays follow BAM's SoA layout: a state is a numpy array of shape
+(nvars, o, n, m) in C order, i.e. one contiguous double[m*n*o] per grid
+function with ijk = i + m*j + m*n*k.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+NV = 25
+VAR_NAMES = ["gxx", "gxy", "gxz", "gyy", "gyz", "gzz", "chi",
+             "Axx", "Axy", "Axz", "Ayy", "Ayz", "Azz", "Khat",
+             "Gx", "Gy", "Gz", "Theta", "alpha",
+             "betax", "betay", "betaz", "Bx", "By", "Bz"]
+
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
+
+
+class Box(C.Structure):
+    _fields_ = [("m", C.c_int), ("n", C.c_int), ("o", C.c_int),
+                ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("skip", C.c_int * 6)]
+
+
+class Z4cParams(C.Structure):
+    _fields_ = [("order_centered", C.c_int), ("order_advection", C.c_int),
+                ("advection_lopsided", C.c_int), ("order_dissipation", C.c_int),
+                ("dissfactor", C.c_double), ("kappa1", C.c_double), ("kappa2", C.c_double),
+                ("zeroThetaRHS", C.c_double), ("subtractA", C.c_int), ("normalizedetg", C.c_int),
+                ("oploglapse", C.c_int), ("oploglapse2", C.c_int), ("oplogwithshift", C.c_int),
+                ("harmoniclapse", C.c_int), ("lapseharmonicf", C.c_double),
+                ("gamma0factor", C.c_double), ("gamma2factor", C.c_double),
+                ("withGadv", C.c_double), ("withShiftadv", C.c_double),
+                ("withBadv", C.c_double), ("withB", C.c_double),
+                ("shiftalphapower", C.c_double), ("shiftgammacoeff", C.c_double),
+                ("shiftdriver", C.c_double), ("chiDivFloor", C.c_double),
+                ("chipsipower", C.c_double)]
+
+
+class C2PParams(C.Structure):
+    _fields_ = [("eos", C.c_int), ("gamma", C.c_double), ("K", C.c_double),
+                ("rhoatm", C.c_double), ("fatm", C.c_double), ("epslatm", C.c_double),
+                ("patm", C.c_double), ("vmax", C.c_double), ("errormax", C.c_double),
+                ("countmax", C.c_int)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        L.ob_z4c_rhs.restype = C.c_int
+        L.ob_z4c_rhs.argtypes = [C.POINTER(Box), C.POINTER(Z4cParams), C.POINTER(DP),
+                                 C.POINTER(DP), C.c_double, C.POINTER(DP)]
+        L.ob_z4c_radiative_bc.argtypes = [C.POINTER(Box), C.POINTER(Z4cParams), C.POINTER(DP),
+                                          C.POINTER(DP), C.c_double, C.POINTER(DP)]
+        L.ob_periodic_fill.argtypes = [C.POINTER(Box), C.c_int * 3, C.c_int, DP]
+        L.ob_z4c_algcon.argtypes = [C.POINTER(Box), C.c_int, C.c_int, C.POINTER(DP), C.POINTER(DP)]
+        L.ob_adm_to_z4c.argtypes = [C.POINTER(Box), C.c_double, C.POINTER(DP), C.POINTER(DP), C.POINTER(DP)]
+        L.ob_z4c_to_adm.argtypes = [C.POINTER(Box), C.c_double, C.POINTER(DP), C.POINTER(DP), C.POINTER(DP)]
+        L.ob_z4c_rk4_step.restype = C.c_int
+        L.ob_z4c_rk4_step.argtypes = [C.POINTER(Box), C.POINTER(Z4cParams), C.c_double, C.c_int,
+                                      C.c_int, C.POINTER(DP), C.POINTER(DP), C.c_int]
+        L.ob_z4c_ham_l2.restype = C.c_double
+        L.ob_z4c_ham_l2.argtypes = [C.POINTER(Box), C.POINTER(DP), C.c_double, DP]
+        L.ob_lagrange_coeffs.argtypes = [C.c_int, C.c_double, DP]
+        L.ob_prolong_1d.argtypes = [C.c_int, C.c_int, DP, DP]
+        L.ob_c2p_hybrid.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP, IP, IP]
+        L.ob_c2p_proot.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP, IP, IP]
+        L.ob_p2c.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP]
+        L.ob_try_hot.restype = C.c_int
+        L.ob_try_hot.argtypes = [C.POINTER(C2PParams), C.c_double, C.c_double, C.c_double,
+                                 DP, DP, DP, C.c_double, IP]
+        _LIB = L
+    return _LIB
+
+
+# --------------------------------------------------------------------------
+# parameters: BAM parameter-file semantics (whole-word Getv) -> struct
+def _getv(value, word):
+    return 1 if word in str(value).split() else 0
+
+
+Z4C_DEFAULTS = {
+    # src/main/main/bam_main.c:54-73, src/projects/z4/bam_z4.c:30-70,
+    # src/main/amr/bam_amr.c:50 (advection_lopsided)
+    "order_centered": 4, "order_advection": 4, "advection_lopsided": 1,
+    "order_dissipation": 4, "dissipation_factor": 0.1,
+    "z4_kappa1": 0.0, "z4_kappa2": 0.0, "z4_zeroThetaRHS": "no",
+    "z4_subtractA": "yes", "z4_normalizedetg": "yes",
+    "z4_lapse": "1+log withshift", "z4_lapseharmonicf": 2.0,
+    "z4_shift": "withShiftadv", "z4_shiftalphapower": 0.0, "z4_shiftgammacoeff": 1.0,
+    "z4_shiftdriver": 2.0, "z4_chi_div_floor": -1000.0, "z4_chi_psipower": -4.0,
+}
+
+
+def z4c_params(**over):
+    d = dict(Z4C_DEFAULTS)
+    d.update(over)
+    p = Z4cParams()
+    p.order_centered = int(d["order_centered"])
+    p.order_advection = int(d["order_advection"])
+    p.advection_lopsided = int(d["advection_lopsided"])
+    p.order_dissipation = int(d["order_dissipation"])
+    p.dissfactor = float(d["dissipation_factor"]) / 2.0 ** p.order_dissipation
+    p.kappa1 = float(d["z4_kappa1"])
+    p.kappa2 = float(d["z4_kappa2"])
+    p.zeroThetaRHS = float(_getv(d["z4_zeroThetaRHS"], "no"))
+    p.subtractA = _getv(d["z4_subtractA"], "yes")
+    p.normalizedetg = _getv(d["z4_normalizedetg"], "yes")
+    p.oploglapse = _getv(d["z4_lapse"], "1+log")
+    p.oploglapse2 = _getv(d["z4_lapse"], "1+log2")
+    p.oplogwithshift = _getv(d["z4_lapse"], "withshift")
+    p.harmoniclapse = _getv(d["z4_lapse"], "harmonic")
+    p.lapseharmonicf = float(d["z4_lapseharmonicf"])
+    p.gamma0factor = float(_getv(d["z4_shift"], "gamma0"))
+    p.gamma2factor = float(_getv(d["z4_shift"], "gamma2"))
+    p.withGadv = float(_getv(d["z4_shift"], "withGadv"))
+    p.withShiftadv = float(_getv(d["z4_shift"], "withShiftadv"))
+    p.withBadv = float(_getv(d["z4_shift"], "withBadv"))
+    p.withB = float(_getv(d["z4_shift"], "withB"))
+    p.shiftalphapower = float(d["z4_shiftalphapower"])
+    p.shiftgammacoeff = float(d["z4_shiftgammacoeff"])
+    p.shiftdriver = float(d["z4_shiftdriver"])
+    p.chiDivFloor = float(d["z4_chi_div_floor"])
+    p.chipsipower = float(d["z4_chi_psipower"])
+    return p
+
+
+def make_box(m, n, o, x0, y0, z0, dx, dy=None, dz=None, skip=(1, 1, 1, 1, 1, 1)):
+    b = Box()
+    b.m, b.n, b.o = m, n, o
+    b.x0, b.y0, b.z0 = x0, y0, z0
+    b.dx = dx
+    b.dy = dx if dy is None else dy
+    b.dz = dx if dz is None else dz
+    for f in range(6):
+        b.skip[f] = skip[f]
+    return b
+
+
+def _ptrs(arr):
+    """array of double* to the leading-axis slices of a C-contiguous array"""
+    assert arr.dtype == np.float64 and arr.flags.c_contiguous
+    nv = arr.shape[0]
+    stride = arr[0].size * 8
+    base = arr.ctypes.data
+    return (DP * nv)(*[C.cast(base + i * stride, DP) for i in range(nv)])
+
+
+def z4c_rhs(box, params, u, unew=None, upre=None, c=0.0):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros_like(u) if unew is None else unew
+    pre = _ptrs(upre) if upre is not None else _ptrs(out)
+    nbad = lib().ob_z4c_rhs(C.byref(box), C.byref(params), _ptrs(out), pre, c, _ptrs(u))
+    return out, nbad
+
+
+def z4c_radiative_bc(box, params, rhs, u):
+    lib().ob_z4c_radiative_bc(C.byref(box), C.byref(params), _ptrs(rhs), _ptrs(rhs), 0.0, _ptrs(u))
+
+
+def periodic_fill(box, u, g, per=(1, 1, 1)):
+    arr = (C.c_int * 3)(*per)
+    for v in range(u.shape[0]):
+        lib().ob_periodic_fill(C.byref(box), arr, g, u[v].ctypes.data_as(DP))
+
+
+def z4c_algcon(box, u, subtractA=1, normalizedetg=1):
+    lib().ob_z4c_algcon(C.byref(box), subtractA, normalizedetg, _ptrs(u[0:6]), _ptrs(u[7:13]))
+
+
+def adm_to_z4c(box, gadm, kadm, chipsipower=-4.0):
+    u = np.zeros((NV,) + gadm.shape[1:])
+    lib().ob_adm_to_z4c(C.byref(box), chipsipower, _ptrs(np.ascontiguousarray(gadm)),
+                        _ptrs(np.ascontiguousarray(kadm)), _ptrs(u))
+    return u
+
+
+def z4c_rk4_step(box, params, dt, u, bc=1, pgh=2, algcon=True, work=None):
+    if work is None:
+        work = np.zeros((3 * NV,) + u.shape[1:])
+    nbad = lib().ob_z4c_rk4_step(C.byref(box), C.byref(params), dt, bc, pgh, _ptrs(u), _ptrs(work),
+                                 1 if algcon else 0)
+    return nbad
+
+
+def z4c_ham_l2(box, u, chipsipower=-4.0, want_field=False):
+    ham = np.zeros(u.shape[1:]) if want_field else None
+    l2 = lib().ob_z4c_ham_l2(C.byref(box), _ptrs(u), chipsipower,
+                             ham.ctypes.data_as(DP) if want_field else None)
+    return (l2, ham) if want_field else l2
+
+
+def lagrange_coeffs(n, x):
+    c = np.zeros(n)
+    lib().ob_lagrange_coeffs(n, x, c.ctypes.data_as(DP))
+    return c
+
+
+def prolong_1d(order, uc):
+    uc = np.ascontiguousarray(uc, dtype=np.float64)
+    uf = np.zeros(2 * uc.size)
+    lib().ob_prolong_1d(order, uc.size, uc.ctypes.data_as(DP), uf.ctypes.data_as(DP))
+    return uf
+
+
+# ---------------------------------------------------------------- con2prim
+C2P_OK, C2P_ATM, C2P_COLD, C2P_FAIL, C2P_NR = 2, 1, 0, 3, 4
+
+
+def c2p_params(eos="ideal", gamma=2.0, K=100.0, rhoatm=1e-20, fatm=100.0, vmax=0.999,
+               errormax=1e-10, countmax=100):
+    P = C2PParams()
+    P.eos = 0 if eos == "ideal" else 1
+    P.gamma, P.K = gamma, K
+    P.rhoatm, P.fatm = rhoatm, fatm
+    # atmosphere from the cold EOS (grhd_atm.c:74-75): polytrope p, eps
+    kr = K * rhoatm ** (gamma - 1.0)
+    P.patm = rhoatm * kr
+    P.epslatm = kr / (gamma - 1.0)
+    P.vmax, P.errormax, P.countmax = vmax, errormax, countmax
+    return P
+
+
+def p2c(P, prim5, metric):
+    prim5 = np.ascontiguousarray(prim5, dtype=np.float64)
+    metric = np.ascontiguousarray(metric, dtype=np.float64)
+    n = prim5.shape[1]
+    cons = np.zeros((5, n))
+    lib().ob_p2c(C.byref(P), n, prim5.ctypes.data_as(DP), metric.ctypes.data_as(DP), cons.ctypes.data_as(DP))
+    return cons
+
+
+def c2p(P, cons, metric, pguess, rhoguess=None, method="hybrid"):
+    """returns (prim[8,n], status[n], iters[n], cons_after[5,n])"""
+    cons = np.array(cons, dtype=np.float64, order="C", copy=True)
+    metric = np.ascontiguousarray(metric, dtype=np.float64)
+    n = cons.shape[1]
+    prim = np.zeros((8, n))
+    prim[5] = pguess
+    if rhoguess is not None:
+        prim[0] = rhoguess
+    st = np.zeros(n, dtype=np.int32)
+    it = np.zeros(n, dtype=np.int32)
+    fn = lib().ob_c2p_hybrid if method == "hybrid" else lib().ob_c2p_proot
+    fn(C.byref(P), n, cons.ctypes.data_as(DP), metric.ctypes.data_as(DP), prim.ctypes.data_as(DP),
+       st.ctypes.data_as(IP), it.ctypes.data_as(IP))
+    return prim, st, it, cons

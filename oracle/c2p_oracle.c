@@ -1,0 +1,351 @@
+/* c2p_oracle.c -- TEST INFRASTRUCTURE: CPU restatement of BAM's GRHD
+ * conservative-to-primitive recovery and its helpers.  Checker only.
+ *
+ * Follows src/projects/grhd/grhd_c2p_v2.c:24-253 (try_hot_part,
+ * try_cold_part), :265-464 (grhd_c2p_proot_ColdStaticAtm_hybrid point loop),
+ * src/projects/grhd/grhd_p2c.c:11-86 (prim -> cons), the analytic EOS of
+ * src/projects/eos/eos_analytic.c:12-72,423-434 and invg of
+ * src/projects/adm/adm_utility.c:27-46.
+ */
+#include <math.h>
+#include "oracle.h"
+
+#define DMAX(a, b) ((a) > (b) ? (a) : (b))
+#define BIG 1e+15
+
+double ob_invg(double g11, double g12, double g13, double g22, double g23, double g33,
+               double *i11, double *i12, double *i13, double *i22, double *i23, double *i33)
+{
+  const double c11 = g22 * g33 - g23 * g23;
+  const double c12 = g13 * g23 - g12 * g33;
+  const double c13 = g12 * g23 - g13 * g22;
+  const double c22 = g11 * g33 - g13 * g13;
+  const double c23 = g12 * g13 - g11 * g23;
+  const double c33 = g11 * g22 - g12 * g12;
+  const double det = g11 * c11 + g12 * c12 + g13 * c13;
+  *i11 = c11 / det;
+  *i12 = c12 / det;
+  *i13 = c13 / det;
+  *i22 = c22 / det;
+  *i23 = c23 / det;
+  *i33 = c33 / det;
+  return det;
+}
+
+/* EOS.comp("re",...,"pc","re"): p(rho,eps) and its partial derivatives.
+ * Ideal gas: p = (G-1) rho eps (eos_ideal); polytrope: p = K rho^G (eos_poly).
+ * Returns the "problem" flag (non-finite input/output). */
+static int eos_re(const ob_c2p_params *P, double rho, double eps, double *p, double *chi, double *kappa)
+{
+  double cs2;
+  if (P->eos == 0) {
+    *p = (P->gamma - 1) * rho * eps;
+    *kappa = (P->gamma - 1) * rho;
+    *chi = (P->gamma - 1) * eps;
+  } else {
+    const double kr = P->K * pow(rho, P->gamma - 1.);
+    *p = rho * kr;
+    *kappa = 0.;
+    *chi = P->gamma * kr;
+  }
+  /* eos_cs2_rep is only used for its finiteness check */
+  cs2 = (rho == 0.) ? 0. : (*chi + *p * *kappa / (rho * rho)) / (1.0 + eps + *p / rho);
+  return (!isfinite(rho) || !isfinite(eps) || !isfinite(cs2)) ? 1 : 0;
+}
+
+/* cold part EOS.comp("r",...,"pe","r"): polytrope with eps reset */
+static void eos_cold(const ob_c2p_params *P, double rho, double *p, double *eps, double *chi)
+{
+  const double kr = P->K * pow(rho, P->gamma - 1.);
+  *p = rho * kr;
+  *chi = P->gamma * kr;
+  *eps = kr / (P->gamma - 1.);
+}
+
+int ob_try_hot(const ob_c2p_params *P, double conD, double conS2, double conT,
+               double *rho, double *epsl, double *pres, double init, int *iters)
+{
+  const double errormax = P->errormax;
+  const int countmax = P->countmax;
+  double pmin = sqrt(conS2) - conD - conT;
+  pmin = DMAX(pmin, P->patm);
+  double pold = DMAX(init, pmin);
+  *iters = 0;
+  if (!isfinite(pold)) return OB_C2P_FAIL;
+  double temp = conT + pold + conD;
+  double tempSQRTarg = temp * temp - conS2;
+  double tempSQRT = sqrt(tempSQRTarg);
+  double Wlor = temp / tempSQRT;
+  *rho = conD / Wlor;
+  *epsl = (tempSQRT - Wlor * pold) / conD - 1.;
+  if (tempSQRTarg <= 0.0) return OB_C2P_COLD;
+
+  int count;
+  double error = 2 * errormax;
+  for (count = 0; count <= countmax; count++) {
+    double peos, chi, kappa;
+    const int problem = eos_re(P, *rho, *epsl, &peos, &chi, &kappa);
+    const double f = pold - peos;
+    const double drhodp = conD * conS2 / (tempSQRT * temp * temp);
+    const double depsldp = pold * conS2 / (conD * tempSQRTarg * tempSQRT);
+    const double df = 1. - chi * drhodp - kappa * depsldp;
+    *pres = DMAX(pmin, pold - f / df);
+    error = fabs(1. - pold / (*pres));
+    pold = *pres;
+    temp = conT + pold + conD;
+    tempSQRTarg = temp * temp - conS2;
+    tempSQRT = sqrt(tempSQRTarg);
+    Wlor = temp / tempSQRT;
+    *rho = conD / Wlor;
+    *epsl = (tempSQRT - Wlor * pold) / conD - 1.;
+    *iters = count + 1;
+    if (*rho < P->fatm * P->rhoatm) return OB_C2P_ATM;
+    if (problem) return OB_C2P_ATM;
+    if (df == 0.) return OB_C2P_NR;
+    if (error < errormax) break;
+  }
+  if (count >= countmax) return OB_C2P_NR;
+  if (!isfinite(*pres) || !isfinite(*rho) || !isfinite(*epsl)) return OB_C2P_COLD;
+  if (*epsl < 0.) return OB_C2P_COLD;
+  const double v2 = conS2 / ((conT + conD + *pres) * (conT + conD + *pres));
+  if (v2 > P->vmax) return OB_C2P_ATM;
+  return OB_C2P_OK;
+}
+
+int ob_try_cold(const ob_c2p_params *P, double conD, double conS2, double *conT,
+                double *rho, double *epsl, double *pres, double init, int *iters)
+{
+  const double errormax = P->errormax;
+  const int countmax = P->countmax;
+  double h, dhdrho, chi, tempSQRTarg, Wlor, g, dg;
+  *iters = 0;
+  if (conD <= 0.) return OB_C2P_ATM;
+  const double rhomin = P->rhoatm;
+  double rhold = DMAX(init, rhomin);
+  if (!isfinite(rhold)) return OB_C2P_FAIL;
+  int count;
+  double error = 2 * errormax;
+  for (count = 0; count <= countmax; count++) {
+    eos_cold(P, rhold, pres, epsl, &chi);
+    h = 1.0 + *epsl + *pres / rhold;
+    dhdrho = chi / rhold;
+    if (conD * h == 0.) tempSQRTarg = 1.0 + conS2 * BIG;
+    else tempSQRTarg = 1.0 + conS2 / ((conD * h) * (conD * h));
+    Wlor = sqrt(tempSQRTarg);
+    g = Wlor * rhold - conD;
+    dg = Wlor - (rhold * conS2 * dhdrho) / (Wlor * conD * conD * h * h * h);
+    *iters = count + 1;
+    if ((!isfinite(dg)) || (dg == 0.0)) return OB_C2P_NR;
+    *rho = DMAX(rhomin, rhold - g / dg);
+    error = fabs(1. - rhold / (*rho));
+    rhold = *rho;
+    if (error < errormax) break;
+  }
+  if (count >= countmax) return OB_C2P_NR;
+  eos_cold(P, rhold, pres, epsl, &chi);
+  h = 1.0 + *epsl + *pres / rhold;
+  if (conD * h == 0.) tempSQRTarg = 1.0 + conS2 * BIG;
+  else tempSQRTarg = 1.0 + conS2 / ((conD * h) * (conD * h));
+  Wlor = sqrt(tempSQRTarg);
+  if (*rho < P->fatm * P->rhoatm) return OB_C2P_ATM;
+  *conT = Wlor * Wlor * (*rho) * h - *pres - conD;
+  const double v2 = conS2 / ((*conT + conD + *pres) * (*conT + conD + *pres));
+  if (v2 > P->vmax) return OB_C2P_ATM;
+  return OB_C2P_OK;
+}
+
+/* one point of grhd_c2p_proot_ColdStaticAtm_hybrid (grhd_c2p_v2.c:303-460);
+ * the NaN diagnostics of the reference only print, so they are omitted
+ * except where they change the control flow (result = FAIL). */
+void ob_c2p_hybrid(const ob_c2p_params *P, long n, double *cons, const double *metric,
+                   double *prim, int *status, int *iters)
+{
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (long q = 0; q < n; q++) {
+    double *MD = cons + 0 * n, *MT = cons + 1 * n, *MSx = cons + 2 * n, *MSy = cons + 3 * n, *MSz = cons + 4 * n;
+    const double *gxx = metric + 0 * n, *gxy = metric + 1 * n, *gxz = metric + 2 * n;
+    const double *gyy = metric + 3 * n, *gyz = metric + 4 * n, *gzz = metric + 5 * n;
+    double *Mrho = prim + 0 * n, *Mepsl = prim + 1 * n, *Mvx = prim + 2 * n, *Mvy = prim + 3 * n;
+    double *Mvz = prim + 4 * n, *Mp = prim + 5 * n, *Mv2 = prim + 6 * n, *detg = prim + 7 * n;
+    double gixx, gixy, gixz, giyy, giyz, gizz;
+    double rho = 0, epsl = 0, pres = 0;
+    int it = 0, it2 = 0;
+
+    detg[q] = ob_invg(gxx[q], gxy[q], gxz[q], gyy[q], gyz[q], gzz[q], &gixx, &gixy, &gixz, &giyy, &giyz, &gizz);
+    double SQRTdetg = sqrt(fabs(detg[q]));
+    if (!isfinite(SQRTdetg)) {
+      detg[q] = 1.;
+      SQRTdetg = 1.;
+    }
+    const double oo = 1. / SQRTdetg;
+    double conD = MD[q] * oo, conSx = MSx[q] * oo, conSy = MSy[q] * oo, conSz = MSz[q] * oo, conT = MT[q] * oo;
+    double conSix = gixx * conSx + gixy * conSy + gixz * conSz;
+    double conSiy = gixy * conSx + giyy * conSy + giyz * conSz;
+    double conSiz = gixz * conSx + giyz * conSy + gizz * conSz;
+    double conS2 = conSx * conSix + conSy * conSiy + conSz * conSiz;
+    double SQRTconS2 = sqrt(fabs(conS2));
+    int result = OB_C2P_OK;
+    if (!isfinite(conD) || !isfinite(conSx) || !isfinite(conSy) || !isfinite(conSz) || !isfinite(conT))
+      result = OB_C2P_FAIL;
+    if (!isfinite(gxx[q]) || !isfinite(gxy[q]) || !isfinite(gxz[q]) || !isfinite(gyy[q]) || !isfinite(gyz[q])
+        || !isfinite(gzz[q]) || !isfinite(detg[q]) || !isfinite(gixx) || !isfinite(gixy) || !isfinite(gixz)
+        || !isfinite(giyy) || !isfinite(giyz) || !isfinite(gizz))
+      result = OB_C2P_FAIL;
+    if (!isfinite(conSix) || !isfinite(conSiy) || !isfinite(conSiz) || !isfinite(conS2) || !isfinite(SQRTconS2)) {
+      conSx = conSix = 0.;
+      conSy = conSiy = 0.;
+      conSz = conSiz = 0.;
+      conS2 = SQRTconS2 = 0.;
+      result = OB_C2P_FAIL;
+    }
+    if (result != OB_C2P_FAIL) result = ob_try_hot(P, conD, conS2, conT, &rho, &epsl, &pres, Mp[q], &it);
+    if (result == OB_C2P_COLD) {
+      result = ob_try_cold(P, conD, conS2, &conT, &rho, &epsl, &pres, Mrho[q], &it2);
+      MT[q] = SQRTdetg * conT;
+    }
+    if (result != OB_C2P_OK && result != OB_C2P_ATM) {
+      /* failures fall back to the atmosphere; the recorded status keeps the cause */
+      status[q] = result;
+      result = OB_C2P_ATM;
+    } else {
+      status[q] = result;
+    }
+    if (result == OB_C2P_ATM) {
+      pres = P->patm;
+      rho = P->rhoatm;
+      epsl = P->epslatm;
+      conD = P->rhoatm;
+      conSx = conSix = 0.;
+      conSy = conSiy = 0.;
+      conSz = conSiz = 0.;
+      conS2 = 0.;
+      conT = P->rhoatm * P->epslatm;
+      MD[q] = SQRTdetg * conD;
+      MSx[q] = SQRTdetg * conSx;
+      MSy[q] = SQRTdetg * conSy;
+      MSz[q] = SQRTdetg * conSz;
+      MT[q] = SQRTdetg * conT;
+    }
+    const double ooDpT = 1.0 / (conD + pres + conT);
+    Mrho[q] = rho;
+    Mepsl[q] = epsl;
+    Mp[q] = pres;
+    Mvx[q] = conSix * ooDpT;
+    Mvy[q] = conSiy * ooDpT;
+    Mvz[q] = conSiz * ooDpT;
+    {
+      const double vlx = gxx[q] * Mvx[q] + gxy[q] * Mvy[q] + gxz[q] * Mvz[q];
+      const double vly = gxy[q] * Mvx[q] + gyy[q] * Mvy[q] + gyz[q] * Mvz[q];
+      const double vlz = gxz[q] * Mvx[q] + gyz[q] * Mvy[q] + gzz[q] * Mvz[q];
+      Mv2[q] = vlx * Mvx[q] + vly * Mvy[q] + vlz * Mvz[q];
+    }
+    iters[q] = it + it2;
+  }
+}
+
+/* prim (rho, eps, vx, vy, vz with contravariant v^i) -> densitised cons
+ * (D, T, Sx, Sy, Sz): grhd_compute_v2_pt + grhd_compute_q_pt, with the
+ * pressure from the EOS and W = 1/sqrt(1-v^2). */
+void ob_p2c(const ob_c2p_params *P, long n, const double *prim5, const double *metric, double *cons)
+{
+  for (long q = 0; q < n; q++) {
+    const double rho = prim5[q], eps = prim5[n + q];
+    const double vx = prim5[2 * n + q], vy = prim5[3 * n + q], vz = prim5[4 * n + q];
+    const double gxx = metric[q], gxy = metric[n + q], gxz = metric[2 * n + q];
+    const double gyy = metric[3 * n + q], gyz = metric[4 * n + q], gzz = metric[5 * n + q];
+    double p, chi, kappa;
+    eos_re(P, rho, eps, &p, &chi, &kappa);
+    const double vlx = gxx * vx + gxy * vy + gxz * vz;
+    const double vly = gxy * vx + gyy * vy + gyz * vz;
+    const double vlz = gxz * vx + gyz * vy + gzz * vz;
+    const double v2 = vlx * vx + vly * vy + vlz * vz;
+    const double W = 1.0 / sqrt(1.0 - v2);
+    double i11, i12, i13, i22, i23, i33;
+    const double detg = ob_invg(gxx, gxy, gxz, gyy, gyz, gzz, &i11, &i12, &i13, &i22, &i23, &i33);
+    const double sq = sqrt(detg);
+    const double W2hrho = W * W * (rho + rho * eps + p);
+    cons[q] = sq * W * rho;
+    cons[n + q] = sq * (W2hrho - p - W * rho);
+    cons[2 * n + q] = sq * W2hrho * vlx;
+    cons[3 * n + q] = sq * W2hrho * vly;
+    cons[4 * n + q] = sq * W2hrho * vlz;
+  }
+}
+
+void ob_c2p_proot(const ob_c2p_params *P, long n, const double *cons, const double *metric,
+                  double *prim, int *status, int *iters)
+{
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (long q = 0; q < n; q++) {
+    const double *gm = metric;
+    double gixx, gixy, gixz, giyy, giyz, gizz;
+    const double detg = ob_invg(gm[q], gm[n + q], gm[2 * n + q], gm[3 * n + q], gm[4 * n + q], gm[5 * n + q],
+                                &gixx, &gixy, &gixz, &giyy, &giyz, &gizz);
+    prim[7 * n + q] = detg;
+    status[q] = OB_C2P_OK;
+    iters[q] = 0;
+    if (detg <= 0. || !isfinite(detg)) {
+      status[q] = OB_C2P_FAIL;
+      continue;
+    }
+    const double oo = 1. / sqrt(detg);
+    const double conD = cons[q] * oo, conT = cons[n + q] * oo;
+    const double conSx = cons[2 * n + q] * oo, conSy = cons[3 * n + q] * oo, conSz = cons[4 * n + q] * oo;
+    const double conSix = gixx * conSx + gixy * conSy + gixz * conSz;
+    const double conSiy = gixy * conSx + giyy * conSy + giyz * conSz;
+    const double conSiz = gixz * conSx + giyz * conSy + gizz * conSz;
+    const double conS2 = conSx * conSix + conSy * conSiy + conSz * conSiz;
+    const double SQRTconS2 = sqrt(conS2);
+    if (!isfinite(conD) || !isfinite(conSx) || !isfinite(conSy) || !isfinite(conSz) || !isfinite(conT)
+        || !isfinite(SQRTconS2)) {
+      status[q] = OB_C2P_FAIL;
+      continue;
+    }
+    const double pmin = SQRTconS2 - conD - conT;
+    double pold = DMAX(prim[5 * n + q], pmin);
+    double temp = conT + pold + conD, arg = temp * temp - conS2, sq = sqrt(arg), W = temp / sq;
+    double rho = conD / W, eps = (sq - W * pold) / conD - 1.;
+    if (!isfinite(pold) || arg < 0.0) {
+      status[q] = OB_C2P_FAIL;
+      continue;
+    }
+    double pnew = pold, error = 2 * P->errormax;
+    int count;
+    for (count = 0; count <= P->countmax; count++) {
+      double peos, chi, kappa;
+      eos_re(P, rho, eps, &peos, &chi, &kappa);
+      const double f = pold - peos;
+      const double drhodp = conD * conS2 / (sq * temp * temp);
+      const double depsldp = pold * conS2 / (conD * arg * sq);
+      const double df = 1. - chi * drhodp - kappa * depsldp;
+      if (df == 0.0) {
+        status[q] = OB_C2P_NR;
+        break;
+      }
+      pnew = DMAX(pmin, pold - f / df);
+      error = fabs(1. - pold / pnew);
+      pold = pnew;
+      temp = conT + pold + conD;
+      arg = temp * temp - conS2;
+      sq = sqrt(arg);
+      W = temp / sq;
+      rho = conD / W;
+      eps = (sq - W * pold) / conD - 1.;
+      iters[q] = count + 1;
+      if (error < P->errormax) break;
+    }
+    if (status[q] == OB_C2P_OK && count >= P->countmax) status[q] = OB_C2P_NR;
+    if (status[q] == OB_C2P_OK && eps < 0.) status[q] = OB_C2P_FAIL;
+    prim[q] = rho;
+    prim[n + q] = eps;
+    prim[5 * n + q] = pnew;
+    prim[2 * n + q] = conSix / (conD + pnew + conT);
+    prim[3 * n + q] = conSiy / (conD + pnew + conT);
+    prim[4 * n + q] = conSiz / (conD + pnew + conT);
+    const double vx = prim[2 * n + q], vy = prim[3 * n + q], vz = prim[4 * n + q];
+    const double vlx = gm[q] * vx + gm[n + q] * vy + gm[2 * n + q] * vz;
+    const double vly = gm[n + q] * vx + gm[3 * n + q] * vy + gm[4 * n + q] * vz;
+    const double vlz = gm[2 * n + q] * vx + gm[4 * n + q] * vy + gm[5 * n + q] * vz;
+    prim[6 * n + q] = vlx * vx + vly * vy + vlz * vz;
+  }
+}

@@ -1,0 +1,141 @@
+/* oracle.h -- CPU restatement of the BAM Z4c / RK4 / FMR / con2prim hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path in
+ * baum_amd/ is compared against.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it; the product path never links it.
+ *
+ * Every function restates the reference algorithm from the reference's own
+ * sources (paths relative to the reference tree, cited at each function) in
+ * plain C written for clarity: full 3x3 index loops, no hand expansion.
+ * Layout mirrors BAM: one double[m*n*o] per grid function component, index
+ * ijk = i + m*j + m*n*k (bam_amr.h:10-31, bam_amr_loops_box.h:117-139).
+ */
+#ifndef BAMHIP_ORACLE_H
+#define BAMHIP_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One BAM box (subset of tB, src/main/amr/bam_amr.h:10-31).  skip[f] is the
+ * number of flagged point layers on face f (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z);
+ * flagged points (PHYBOUND/GHOBOUND/PERBOUND/REFBOUND) are not touched by
+ * the RHS loop, which in BAM is forinnerpoints_ijk_openmp = !boundary[ijk]. */
+typedef struct {
+  int m, n, o;
+  double x0, y0, z0;
+  double dx, dy, dz;
+  int skip[6];
+} ob_box;
+
+/* Snapshot of the parameters z4_rhs_movpunc_N reads on every call
+ * (src/projects/z4/z4_rhs_movpunc_N.c:108-142, defaults bam_z4.c:30-70). */
+typedef struct {
+  int order_centered;      /* 2 or 4 */
+  int order_advection;     /* 2 or 4 */
+  int advection_lopsided;  /* 0,1,2 */
+  int order_dissipation;   /* 0 or 4 */
+  double dissfactor;       /* get_dissipation_factor(): sigma / 2^order */
+  double kappa1, kappa2;
+  double zeroThetaRHS;     /* 1 keeps the Theta RHS (Getv(...,"no") = 1) */
+  int subtractA, normalizedetg;
+  int oploglapse, oploglapse2, oplogwithshift, harmoniclapse;
+  double lapseharmonicf;
+  double gamma0factor, gamma2factor, withGadv, withShiftadv, withBadv, withB;
+  double shiftalphapower, shiftgammacoeff, shiftdriver;
+  double chiDivFloor, chipsipower;
+} ob_z4c_params;
+
+/* variable order of the Z4c evolution list (z4.c:234-242) */
+enum {
+  OV_GXX = 0, OV_GXY, OV_GXZ, OV_GYY, OV_GYZ, OV_GZZ,
+  OV_CHI,
+  OV_AXX, OV_AXY, OV_AXZ, OV_AYY, OV_AYZ, OV_AZZ,
+  OV_KHAT,
+  OV_GX, OV_GY, OV_GZ,
+  OV_THETA,
+  OV_ALPHA,
+  OV_BETAX, OV_BETAY, OV_BETAZ,
+  OV_BX, OV_BY, OV_BZ,
+  OV_NVARS
+};
+
+/* Z4c right-hand side on one box; returns the number of inner points whose
+ * RHS was not finite (the reference errorexits on the first such point). */
+int ob_z4c_rhs(const ob_box *b, const ob_z4c_params *p,
+               double *const *unew, const double *const *upre, double c,
+               const double *const *ucur);
+
+/* radiative (Sommerfeld) condition on the PHYBOUND layer of one variable */
+void ob_radiative_bc(const ob_box *b, double *unew, const double *upre, double c,
+                     const double *ucur, double var0, double v);
+/* all 25 Z4c variables with the far limits / speeds of z4_startup */
+void ob_z4c_radiative_bc(const ob_box *b, const ob_z4c_params *p,
+                         double *const *unew, const double *const *upre, double c,
+                         const double *const *ucur);
+/* BAM-style periodic copy of g layers in the directions with per[d] != 0 */
+void ob_periodic_fill(const ob_box *b, const int per[3], int g, double *u);
+
+/* algebraic constraints (z4_algcon.m) on all points of one box */
+void ob_z4c_algcon(const ob_box *b, int subtractA, int normalizedetg,
+                   double *const *g6, double *const *A6);
+
+/* ADM -> Z4c (z4_init.m) and Z4c -> ADM (z4.c:93-124) */
+void ob_adm_to_z4c(const ob_box *b, double chipsipower,
+                   const double *const *gadm, const double *const *kadm,
+                   double *const *u);
+void ob_z4c_to_adm(const ob_box *b, double chipsipower, const double *const *u,
+                   double *const *gadm, double *const *kadm);
+
+/* full RK4 step on one box (rk.c:220-257 + z4_algcon_wrapper) with either
+ * radiative (bc=1) or periodic (bc=2, g layers) boundaries.  u holds the
+ * state (25 fields), work must hold 3*25 fields of the same size. */
+int ob_z4c_rk4_step(const ob_box *b, const ob_z4c_params *p, double dt, int bc,
+                    int pgh, double *const *u, double *const *work, int algcon);
+
+/* vector ops of variables.c:575-745 on npts points */
+void ob_vladd(long npts, double *r, double ca, const double *a, double cb, const double *b);
+
+/* Hamiltonian constraint of the Z4c data (via ADM, adm_constraints_N order 4)
+ * at inner points; returns the L2 norm sqrt(sum H^2 / n) over them */
+double ob_z4c_ham_l2(const ob_box *b, const double *const *u, double chipsipower, double *ham);
+
+/* ---------------- interpolation (src/utility/interpolate) ----------------- */
+void ob_lagrange_coeffs(int n, double x, double *c);
+void ob_prolong_1d(int order, int nc, const double *uc, double *uf);
+
+/* ---------------- con2prim (src/projects/grhd) ---------------------------- */
+typedef struct {
+  int eos;               /* 0 ideal gas (cold part polytrope), 1 polytrope */
+  double gamma, K;       /* Gamma and K of the (cold) polytrope / ideal gas */
+  double rhoatm, fatm, epslatm, patm;
+  double vmax;           /* grhd_vmax (HRSC_VMAX) */
+  double errormax;       /* grhd_C2P_NewtonRaphsonTR */
+  int countmax;          /* grhd_C2P_NewtonRaphsonNR */
+} ob_c2p_params;
+
+enum { OB_C2P_COLD = 0, OB_C2P_ATM, OB_C2P_OK, OB_C2P_FAIL, OB_C2P_NR };
+
+double ob_invg(double gxx, double gxy, double gxz, double gyy, double gyz, double gzz,
+               double *ixx, double *ixy, double *ixz, double *iyy, double *iyz, double *izz);
+int ob_try_hot(const ob_c2p_params *P, double D, double S2, double T,
+               double *rho, double *eps, double *p, double init, int *iters);
+int ob_try_cold(const ob_c2p_params *P, double D, double S2, double *T,
+                double *rho, double *eps, double *p, double init, int *iters);
+/* point loop of grhd_c2p_proot_ColdStaticAtm_hybrid.  cons: 5*n (D,T,Sx,Sy,Sz,
+ * densitised, rewritten on ATM/COLD), metric: 6*n, prim: 8*n (rho,eps,vx,vy,vz,
+ * p,v2,detg; p and rho hold the previous values on entry), status/iters: n */
+void ob_c2p_hybrid(const ob_c2p_params *P, long n, double *cons, const double *metric,
+                   double *prim, int *status, int *iters);
+/* prim -> cons (grhd_p2c.c:11-86) */
+void ob_p2c(const ob_c2p_params *P, long n, const double *prim5, const double *metric,
+            double *cons);
+/* plain p-root (grhd_c2p.c:102-237): no atmosphere; the reference's
+ * errorexit cases become status FAIL (bad input) or NR (no convergence) */
+void ob_c2p_proot(const ob_c2p_params *P, long n, const double *cons, const double *metric,
+                  double *prim, int *status, int *iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
