@@ -1,0 +1,308 @@
+"""baum_amd: Python view of the MI355X (gfx950) BAM hot-path library.
+
+The product is libbamhip.so (HIP kernels + C host drivers) behind the C ABI
+declared in include/bamhip.h.  This module binds that ABI with ctypes so the
+tests and the benchmark can drive it with numpy arrays; it holds no compute
+of its own, and every entry point fails loudly if the library is missing.
+
+Names follow the reference: a Level mirrors BAM's tL (boxes concatenated in
+one index space, one device array per variable index), variable lists are
+lists of variable indices as in tVarList, and parameters are given with the
+reference's parameter-file names (bam_z4.c, bam_main.c).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libbamhip.so")
+_LIB = None
+
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
+
+NV = 25
+VAR_NAMES = ["gxx", "gxy", "gxz", "gyy", "gyz", "gzz", "chi",
+             "Axx", "Axy", "Axz", "Ayy", "Ayz", "Azz", "Khat",
+             "Gx", "Gy", "Gz", "Theta", "alpha",
+             "betax", "betay", "betaz", "Bx", "By", "Bz"]
+
+REFBOUND, NOTBOUND, PHYBOUND, GHOBOUND, SYMBOUND, PERBOUND = -1, 0, 1, 2, 3, 6
+
+
+class BamhipError(RuntimeError):
+    pass
+
+
+class Box(C.Structure):
+    _fields_ = [("m", C.c_int), ("n", C.c_int), ("o", C.c_int),
+                ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("skip", C.c_int * 6), ("bflag", C.c_int * 6)]
+
+
+class Z4cParams(C.Structure):
+    _fields_ = [("order_centered", C.c_int), ("order_advection", C.c_int),
+                ("advection_lopsided", C.c_int), ("order_dissipation", C.c_int),
+                ("dissfactor", C.c_double), ("kappa1", C.c_double), ("kappa2", C.c_double),
+                ("zeroThetaRHS", C.c_double), ("subtractA", C.c_int), ("normalizedetg", C.c_int),
+                ("oploglapse", C.c_int), ("oploglapse2", C.c_int), ("oplogwithshift", C.c_int),
+                ("harmoniclapse", C.c_int), ("lapseharmonicf", C.c_double),
+                ("gamma0factor", C.c_double), ("gamma2factor", C.c_double),
+                ("withGadv", C.c_double), ("withShiftadv", C.c_double),
+                ("withBadv", C.c_double), ("withB", C.c_double),
+                ("shiftalphapower", C.c_double), ("shiftgammacoeff", C.c_double),
+                ("shiftdriver", C.c_double), ("chiDivFloor", C.c_double),
+                ("chipsipower", C.c_double)]
+
+
+def _sig(L, name, res, args):
+    f = getattr(L, name)
+    f.restype = res
+    f.argtypes = args
+
+
+def lib():
+    """load libbamhip.so (raises if it is not built)"""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise BamhipError("baum_amd/libbamhip.so is not built; run __graft_entry__.build()")
+    L = C.CDLL(LIBPATH)
+    i, d, vp = C.c_int, C.c_double, C.c_void_p
+    _sig(L, "bamhip_init", i, [i])
+    _sig(L, "bamhip_finalize", i, [])
+    _sig(L, "bamhip_last_error", C.c_char_p, [])
+    _sig(L, "bamhip_stream", vp, [])
+    _sig(L, "bamhip_synchronize", i, [])
+    _sig(L, "bamhip_malloc", i, [C.POINTER(vp), C.c_size_t])
+    _sig(L, "bamhip_free", i, [vp])
+    _sig(L, "bamhip_memcpy_htod", i, [vp, vp, C.c_size_t])
+    _sig(L, "bamhip_memcpy_dtoh", i, [vp, vp, C.c_size_t])
+    _sig(L, "bamhip_level_create", i, [C.POINTER(vp), i, i, C.POINTER(Box), i])
+    _sig(L, "bamhip_level_free", i, [vp])
+    _sig(L, "bamhip_level_npoints", i, [vp, C.POINTER(C.c_long)])
+    _sig(L, "bamhip_level_enable", i, [vp, i])
+    _sig(L, "bamhip_level_disable", i, [vp, i])
+    _sig(L, "bamhip_level_ptr", vp, [vp, i])
+    _sig(L, "bamhip_level_put", i, [vp, i, DP])
+    _sig(L, "bamhip_level_get", i, [vp, i, DP])
+    _sig(L, "bamhip_level_fill", i, [vp, i, d])
+    _sig(L, "bamhip_level_set_check", i, [vp, i])
+    _sig(L, "bamhip_level_nonfinite", i, [vp, i, IP])
+    _sig(L, "bamhip_z4c_rhs", i, [vp, C.POINTER(Z4cParams), IP, IP, d, IP])
+    _sig(L, "bamhip_z4c_set_boundary", i, [vp, C.POINTER(Z4cParams), IP, IP, d, IP, i])
+    _sig(L, "bamhip_vladd", i, [vp, i, IP, d, IP, d, IP])
+    _sig(L, "bamhip_rk4_stage_update", i, [vp, i, IP, IP, IP, IP, d, d])
+    _sig(L, "bamhip_z4c_rk4_step", i, [vp, C.POINTER(Z4cParams), d, IP, IP, IP, IP, i, i])
+    _sig(L, "bamhip_z4c_algcon", i, [vp, IP, IP, i, i])
+    _sig(L, "bamhip_z4c_ham_l2", i, [vp, IP, d, i, DP])
+    _sig(L, "bamhip_timer_enable", i, [i])
+    _sig(L, "bamhip_timer_reset", i, [])
+    _sig(L, "bamhip_timer_get", i, [C.c_char_p, DP, C.POINTER(C.c_long)])
+    _LIB = L
+    return L
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().bamhip_last_error().decode(errors="replace")
+        raise BamhipError("%s failed (%d): %s" % (what, rc, msg))
+    return rc
+
+
+_INIT = False
+
+
+def init(device=0):
+    global _INIT
+    if not _INIT:
+        check(lib().bamhip_init(device), "bamhip_init")
+        _INIT = True
+
+
+def synchronize():
+    check(lib().bamhip_synchronize(), "bamhip_synchronize")
+
+
+# ---------------------------------------------------------------- parameters
+def _getv(value, word):
+    """BAM's Getv: whole-word membership in a parameter value list
+    (src/main/main/parameters.c:883-910)"""
+    return 1 if word in str(value).split() else 0
+
+
+Z4C_DEFAULTS = {
+    "order_centered": 4, "order_advection": 4, "advection_lopsided": 1,
+    "order_dissipation": 4, "dissipation_factor": 0.1,
+    "z4_kappa1": 0.0, "z4_kappa2": 0.0, "z4_zeroThetaRHS": "no",
+    "z4_subtractA": "yes", "z4_normalizedetg": "yes",
+    "z4_lapse": "1+log withshift", "z4_lapseharmonicf": 2.0,
+    "z4_shift": "withShiftadv", "z4_shiftalphapower": 0.0, "z4_shiftgammacoeff": 1.0,
+    "z4_shiftdriver": 2.0, "z4_chi_div_floor": -1000.0, "z4_chi_psipower": -4.0,
+}
+
+
+def z4c_params(**over):
+    """parameter snapshot from BAM parameter names (defaults: the puncture
+    configuration used throughout, 1+log withshift, Gamma driver eta=2, KO 0.1)"""
+    dd = dict(Z4C_DEFAULTS)
+    dd.update(over)
+    p = Z4cParams()
+    p.order_centered = int(dd["order_centered"])
+    p.order_advection = int(dd["order_advection"])
+    p.advection_lopsided = int(dd["advection_lopsided"])
+    p.order_dissipation = int(dd["order_dissipation"])
+    p.dissfactor = float(dd["dissipation_factor"]) / 2.0 ** p.order_dissipation
+    p.kappa1, p.kappa2 = float(dd["z4_kappa1"]), float(dd["z4_kappa2"])
+    p.zeroThetaRHS = float(_getv(dd["z4_zeroThetaRHS"], "no"))
+    p.subtractA = _getv(dd["z4_subtractA"], "yes")
+    p.normalizedetg = _getv(dd["z4_normalizedetg"], "yes")
+    p.oploglapse = _getv(dd["z4_lapse"], "1+log")
+    p.oploglapse2 = _getv(dd["z4_lapse"], "1+log2")
+    p.oplogwithshift = _getv(dd["z4_lapse"], "withshift")
+    p.harmoniclapse = _getv(dd["z4_lapse"], "harmonic")
+    p.lapseharmonicf = float(dd["z4_lapseharmonicf"])
+    p.gamma0factor = float(_getv(dd["z4_shift"], "gamma0"))
+    p.gamma2factor = float(_getv(dd["z4_shift"], "gamma2"))
+    p.withGadv = float(_getv(dd["z4_shift"], "withGadv"))
+    p.withShiftadv = float(_getv(dd["z4_shift"], "withShiftadv"))
+    p.withBadv = float(_getv(dd["z4_shift"], "withBadv"))
+    p.withB = float(_getv(dd["z4_shift"], "withB"))
+    p.shiftalphapower = float(dd["z4_shiftalphapower"])
+    p.shiftgammacoeff = float(dd["z4_shiftgammacoeff"])
+    p.shiftdriver = float(dd["z4_shiftdriver"])
+    p.chiDivFloor = float(dd["z4_chi_div_floor"])
+    p.chipsipower = float(dd["z4_chi_psipower"])
+    return p
+
+
+def make_box(m, n, o, x0, y0, z0, dx, dy=None, dz=None, skip=(1,) * 6, bflag=(PHYBOUND,) * 6):
+    b = Box()
+    b.m, b.n, b.o = m, n, o
+    b.x0, b.y0, b.z0 = x0, y0, z0
+    b.dx = dx
+    b.dy = dx if dy is None else dy
+    b.dz = dx if dz is None else dz
+    for f in range(6):
+        b.skip[f] = skip[f]
+        b.bflag[f] = bflag[f]
+    return b
+
+
+def _iarr(lst):
+    return (C.c_int * len(lst))(*[int(x) for x in lst])
+
+
+class Level:
+    """device mirror of one BAM level (tL) with `nvariables` variable slots"""
+
+    def __init__(self, boxes, nvariables, l=0):
+        init()
+        self.boxes = list(boxes)
+        arr = (Box * len(self.boxes))(*self.boxes)
+        h = C.c_void_p()
+        check(lib().bamhip_level_create(C.byref(h), l, len(self.boxes), arr, nvariables), "bamhip_level_create")
+        self.h = h
+        n = C.c_long()
+        check(lib().bamhip_level_npoints(self.h, C.byref(n)), "bamhip_level_npoints")
+        self.npoints = n.value
+        self.nvariables = nvariables
+        self.offsets = []
+        off = 0
+        for b in self.boxes:
+            self.offsets.append(off)
+            off += b.m * b.n * b.o
+
+    def free(self):
+        if self.h:
+            lib().bamhip_level_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def enable(self, vars_):
+        for v in vars_:
+            check(lib().bamhip_level_enable(self.h, int(v)), "bamhip_level_enable")
+
+    def put(self, var, data):
+        a = np.ascontiguousarray(data, dtype=np.float64).reshape(-1)
+        assert a.size == self.npoints, (a.size, self.npoints)
+        check(lib().bamhip_level_put(self.h, int(var), a.ctypes.data_as(DP)), "bamhip_level_put")
+
+    def get(self, var):
+        a = np.empty(self.npoints)
+        check(lib().bamhip_level_get(self.h, int(var), a.ctypes.data_as(DP)), "bamhip_level_get")
+        return a
+
+    def fill(self, var, value):
+        check(lib().bamhip_level_fill(self.h, int(var), float(value)), "bamhip_level_fill")
+
+    def put_state(self, idx, fields):
+        """fields: array (len(idx), npoints) or list of per-box (nv, o, n, m)"""
+        for k, v in enumerate(idx):
+            self.put(v, fields[k])
+
+    def get_state(self, idx):
+        return np.array([self.get(v) for v in idx])
+
+    def box_view(self, flat, b):
+        bx = self.boxes[b]
+        o = self.offsets[b]
+        return flat[..., o:o + bx.m * bx.n * bx.o].reshape(flat.shape[:-1] + (bx.o, bx.n, bx.m))
+
+    # -- Z4c
+    def z4c_rhs(self, params, unew, ucur, upre=None, c=0.0):
+        check(lib().bamhip_z4c_rhs(self.h, C.byref(params), _iarr(unew), _iarr(upre) if upre is not None else None,
+                                   c, _iarr(ucur)), "bamhip_z4c_rhs")
+
+    def z4c_set_boundary(self, params, unew, ucur, upre=None, c=0.0, pgh=0):
+        check(lib().bamhip_z4c_set_boundary(self.h, C.byref(params), _iarr(unew),
+                                            _iarr(upre) if upre is not None else None, c, _iarr(ucur), pgh),
+              "bamhip_z4c_set_boundary")
+
+    def vladd(self, r, ca, a, cb, b):
+        check(lib().bamhip_vladd(self.h, len(r), _iarr(r), ca, _iarr(a) if a is not None else None, cb,
+                                 _iarr(b) if b is not None else None), "bamhip_vladd")
+
+    def z4c_rk4_step(self, params, dt, uc, up, r, v, pgh=0, algcon=True):
+        check(lib().bamhip_z4c_rk4_step(self.h, C.byref(params), dt, _iarr(uc), _iarr(up), _iarr(r), _iarr(v),
+                                        pgh, 1 if algcon else 0), "bamhip_z4c_rk4_step")
+
+    def z4c_algcon(self, g, A, subtractA=1, normalizedetg=1):
+        check(lib().bamhip_z4c_algcon(self.h, _iarr(g), _iarr(A), subtractA, normalizedetg), "bamhip_z4c_algcon")
+
+    def z4c_ham_l2(self, u, chipsipower=-4.0, ham=-1):
+        out = C.c_double()
+        check(lib().bamhip_z4c_ham_l2(self.h, _iarr(u), chipsipower, ham, C.byref(out)), "bamhip_z4c_ham_l2")
+        return out.value
+
+
+# ------------------------------------------------------------------ timers
+def timer_enable(on=True):
+    check(lib().bamhip_timer_enable(1 if on else 0), "bamhip_timer_enable")
+
+
+def timer_reset():
+    check(lib().bamhip_timer_reset(), "bamhip_timer_reset")
+
+
+def timer_get(name):
+    ms = C.c_double()
+    n = C.c_long()
+    check(lib().bamhip_timer_get(name.encode(), C.byref(ms), C.byref(n)), "bamhip_timer_get")
+    return ms.value, n.value
+
+
+def gpu_available():
+    """True when a HIP device is visible (does not initialise the library)"""
+    try:
+        return lib().bamhip_init(0) == 0
+    except Exception:
+        return False

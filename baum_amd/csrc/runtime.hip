@@ -1,0 +1,294 @@
+// runtime.hip -- device selection, stream, error state, level mirrors, timers.
+#include <cstdarg>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+#include "bamhip_internal.h"
+
+namespace bamhip {
+
+static thread_local char g_err[1024] = "";
+static hipStream_t g_stream = nullptr;
+static int g_device = -1;
+
+int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int check_hip(hipError_t e, const char *what) {
+  return set_error(BAMHIP_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+hipStream_t stream() { return g_stream; }
+
+int gather_ptrs(bamhip_level *lev, const int *idx, int n, double **out, const char *what) {
+  if (!idx) return set_error(BAMHIP_EINVAL, "%s: variable list is NULL", what);
+  for (int i = 0; i < n; i++) {
+    const int k = idx[i];
+    if (k < 0 || k >= lev->nvariables || !lev->v[k])
+      return set_error(BAMHIP_EINVAL, "%s: variable index %d (list entry %d) is not enabled", what, k, i);
+    out[i] = lev->v[k];
+  }
+  return BAMHIP_OK;
+}
+
+// ------------------------------------------------------------------ timers
+struct TimerRec {
+  double ms = 0;
+  long launches = 0;
+};
+static bool g_timer_on = false;
+static std::map<std::string, TimerRec> g_timers;
+static std::mutex g_timer_mu;
+
+TimerScope::TimerScope(const char *name) : name_(name), a_(nullptr), b_(nullptr), on_(g_timer_on) {
+  if (!on_) return;
+  if (hipEventCreate(&a_) != hipSuccess || hipEventCreate(&b_) != hipSuccess) {
+    on_ = false;
+    return;
+  }
+  (void)hipEventRecord(a_, g_stream);
+}
+
+TimerScope::~TimerScope() {
+  if (!on_) return;
+  (void)hipEventRecord(b_, g_stream);
+  (void)hipEventSynchronize(b_);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, a_, b_);
+  {
+    std::lock_guard<std::mutex> lk(g_timer_mu);
+    TimerRec &r = g_timers[name_];
+    r.ms += ms;
+    r.launches += 1;
+  }
+  (void)hipEventDestroy(a_);
+  (void)hipEventDestroy(b_);
+}
+
+}  // namespace bamhip
+
+using namespace bamhip;
+
+extern "C" {
+
+int bamhip_init(int device) {
+  int n = 0;
+  BH_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return set_error(BAMHIP_EINVAL, "bamhip_init: device %d of %d", device, n);
+  BH_HIP(hipSetDevice(device));
+  if (!g_stream || g_device != device) {
+    if (g_stream) (void)hipStreamDestroy(g_stream);
+    BH_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    g_device = device;
+  }
+  return BAMHIP_OK;
+}
+
+int bamhip_finalize(void) {
+  if (g_stream) {
+    BH_HIP(hipStreamSynchronize(g_stream));
+    BH_HIP(hipStreamDestroy(g_stream));
+    g_stream = nullptr;
+  }
+  g_device = -1;
+  return BAMHIP_OK;
+}
+
+const char *bamhip_last_error(void) { return g_err; }
+
+void *bamhip_stream(void) { return (void *)g_stream; }
+
+int bamhip_synchronize(void) {
+  BH_HIP(hipStreamSynchronize(g_stream));
+  return BAMHIP_OK;
+}
+
+int bamhip_malloc(void **ptr, size_t bytes) {
+  if (!ptr) return set_error(BAMHIP_EINVAL, "bamhip_malloc: NULL");
+  BH_HIP(hipMalloc(ptr, bytes ? bytes : 8));
+  return BAMHIP_OK;
+}
+
+int bamhip_free(void *ptr) {
+  if (ptr) BH_HIP(hipFree(ptr));
+  return BAMHIP_OK;
+}
+
+int bamhip_memcpy_htod(void *dst, const void *src, size_t bytes) {
+  BH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+  BH_HIP(hipStreamSynchronize(g_stream));
+  return BAMHIP_OK;
+}
+
+int bamhip_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+  BH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+  BH_HIP(hipStreamSynchronize(g_stream));
+  return BAMHIP_OK;
+}
+
+int bamhip_level_create(bamhip_level **lev, int l, int nboxes, const bamhip_box *boxes, int nvariables) {
+  if (!lev || nboxes <= 0 || !boxes || nvariables <= 0)
+    return set_error(BAMHIP_EINVAL, "bamhip_level_create: bad arguments");
+  if (!g_stream) return set_error(BAMHIP_EINVAL, "bamhip_level_create: call bamhip_init first");
+  bamhip_level *L = new bamhip_level();
+  L->l = l;
+  L->nboxes = nboxes;
+  L->nvariables = nvariables;
+  L->v.assign(nvariables, nullptr);
+  long off = 0;
+  for (int b = 0; b < nboxes; b++) {
+    const bamhip_box &B = boxes[b];
+    if (B.m < 1 || B.n < 1 || B.o < 1 || !(B.dx > 0) || !(B.dy > 0) || !(B.dz > 0)) {
+      delete L;
+      return set_error(BAMHIP_EINVAL, "bamhip_level_create: box %d has bad size or spacing", b);
+    }
+    for (int f = 0; f < 6; f++)
+      if (B.skip[f] < 1 || B.skip[f] > 16) {
+        delete L;
+        return set_error(BAMHIP_EINVAL,
+                         "bamhip_level_create: box %d face %d skip=%d (the outermost layer is always "
+                         "flagged in BAM, and at most 16 layers are supported)",
+                         b, f, B.skip[f]);
+      }
+    DevBox D;
+    D.m = B.m;
+    D.n = B.n;
+    D.o = B.o;
+    D.noffset = off;
+    D.x0 = B.x0;
+    D.y0 = B.y0;
+    D.z0 = B.z0;
+    D.dx = B.dx;
+    D.dy = B.dy;
+    D.dz = B.dz;
+    for (int f = 0; f < 6; f++) {
+      D.skip[f] = B.skip[f];
+      D.bflag[f] = B.bflag[f];
+    }
+    L->boxes.push_back(D);
+    off += (long)B.m * B.n * B.o;
+  }
+  L->npoints = off;
+  if (hipMalloc(&L->d_flag, sizeof(int)) != hipSuccess) {
+    delete L;
+    return set_error(BAMHIP_ENOMEM, "bamhip_level_create: flag allocation failed");
+  }
+  if (hipMemsetAsync(L->d_flag, 0, sizeof(int), g_stream) != hipSuccess ||
+      hipStreamSynchronize(g_stream) != hipSuccess) {
+    (void)hipFree(L->d_flag);
+    delete L;
+    return set_error(BAMHIP_EDEVICE, "bamhip_level_create: flag initialisation failed");
+  }
+  *lev = L;
+  return BAMHIP_OK;
+}
+
+int bamhip_level_free(bamhip_level *lev) {
+  if (!lev) return BAMHIP_OK;
+  if (g_stream) (void)hipStreamSynchronize(g_stream);
+  for (double *p : lev->v)
+    if (p) (void)hipFree(p);
+  if (lev->d_flag) (void)hipFree(lev->d_flag);
+  delete lev;
+  return BAMHIP_OK;
+}
+
+int bamhip_level_npoints(const bamhip_level *lev, long *np) {
+  if (!lev || !np) return set_error(BAMHIP_EINVAL, "bamhip_level_npoints: NULL");
+  *np = lev->npoints;
+  return BAMHIP_OK;
+}
+
+int bamhip_level_enable(bamhip_level *lev, int var) {
+  if (!lev || var < 0 || var >= lev->nvariables)
+    return set_error(BAMHIP_EINVAL, "bamhip_level_enable: bad variable index %d", var);
+  if (lev->v[var]) return BAMHIP_OK;
+  double *p = nullptr;
+  if (hipMalloc(&p, sizeof(double) * lev->npoints) != hipSuccess)
+    return set_error(BAMHIP_ENOMEM, "bamhip_level_enable: out of device memory (%ld points)", lev->npoints);
+  lev->v[var] = p;
+  return BAMHIP_OK;
+}
+
+int bamhip_level_disable(bamhip_level *lev, int var) {
+  if (!lev || var < 0 || var >= lev->nvariables)
+    return set_error(BAMHIP_EINVAL, "bamhip_level_disable: bad variable index %d", var);
+  if (lev->v[var]) {
+    BH_HIP(hipStreamSynchronize(g_stream));
+    BH_HIP(hipFree(lev->v[var]));
+    lev->v[var] = nullptr;
+  }
+  return BAMHIP_OK;
+}
+
+double *bamhip_level_ptr(bamhip_level *lev, int var) {
+  if (!lev || var < 0 || var >= lev->nvariables) return nullptr;
+  return lev->v[var];
+}
+
+int bamhip_level_put(bamhip_level *lev, int var, const double *host) {
+  double *p = bamhip_level_ptr(lev, var);
+  if (!p || !host) return set_error(BAMHIP_EINVAL, "bamhip_level_put: variable %d not enabled", var);
+  BH_HIP(hipMemcpyAsync(p, host, sizeof(double) * lev->npoints, hipMemcpyHostToDevice, g_stream));
+  BH_HIP(hipStreamSynchronize(g_stream));
+  return BAMHIP_OK;
+}
+
+int bamhip_level_get(bamhip_level *lev, int var, double *host) {
+  double *p = bamhip_level_ptr(lev, var);
+  if (!p || !host) return set_error(BAMHIP_EINVAL, "bamhip_level_get: variable %d not enabled", var);
+  BH_HIP(hipMemcpyAsync(host, p, sizeof(double) * lev->npoints, hipMemcpyDeviceToHost, g_stream));
+  BH_HIP(hipStreamSynchronize(g_stream));
+  return BAMHIP_OK;
+}
+
+__global__ void set_flag_kernel(int *p, int v) { *p = v; }
+
+int bamhip_level_set_check(bamhip_level *lev, int mode) {
+  if (!lev) return set_error(BAMHIP_EINVAL, "bamhip_level_set_check: NULL level");
+  lev->check_mode = mode ? 1 : 0;
+  return BAMHIP_OK;
+}
+
+int bamhip_level_nonfinite(bamhip_level *lev, int reset, int *flag) {
+  if (!lev) return set_error(BAMHIP_EINVAL, "bamhip_level_nonfinite: NULL level");
+  if (flag) {
+    int h = 0;
+    BH_HIP(hipMemcpyAsync(&h, lev->d_flag, sizeof(int), hipMemcpyDeviceToHost, g_stream));
+    BH_HIP(hipStreamSynchronize(g_stream));
+    *flag = h;
+  }
+  if (reset) {
+    set_flag_kernel<<<1, 1, 0, g_stream>>>(lev->d_flag, 0);
+    BH_HIP(hipGetLastError());
+  }
+  return BAMHIP_OK;
+}
+
+int bamhip_timer_enable(int on) {
+  g_timer_on = on != 0;
+  return BAMHIP_OK;
+}
+
+int bamhip_timer_reset(void) {
+  std::lock_guard<std::mutex> lk(g_timer_mu);
+  g_timers.clear();
+  return BAMHIP_OK;
+}
+
+int bamhip_timer_get(const char *name, double *ms_total, long *launches) {
+  if (!name) return set_error(BAMHIP_EINVAL, "bamhip_timer_get: NULL name");
+  std::lock_guard<std::mutex> lk(g_timer_mu);
+  auto it = g_timers.find(name);
+  if (ms_total) *ms_total = (it == g_timers.end()) ? 0.0 : it->second.ms;
+  if (launches) *launches = (it == g_timers.end()) ? 0 : it->second.launches;
+  return BAMHIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,478 @@
+// z4c_aux.hip -- the streaming and boundary kernels around the Z4c RHS:
+// SoA vector ops and the fused RK4 stage update, the radiative and periodic
+// boundaries, the algebraic constraints and the Hamiltonian-constraint norm.
+#include <cmath>
+#include <vector>
+
+#include "bamhip_internal.h"
+
+namespace bamhip {
+
+int z4c_rhs_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, const int *upre, double c,
+                 const int *ucur, bool check_finite);
+
+struct Ptrs25 {
+  double *p[BAMHIP_Z4C_NVARS];
+};
+
+// --------------------------------------------------------------- streaming
+__global__ void fill_kernel(double *__restrict__ p, long n, double v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// r = ca*a + cb*b for nv variables (vladd, variables.c:650-719); 2D grid:
+// blockIdx.y selects the variable, the x dimension strides over points
+__global__ void vladd_kernel(Ptrs25 r, Ptrs25 a, Ptrs25 b, double ca, double cb, long n) {
+  const int v = blockIdx.y;
+  double *__restrict__ rv = r.p[v];
+  const double *__restrict__ av = a.p[v];
+  const double *__restrict__ bv = b.p[v];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    if (ca != 0.0) s += ca * av[i];
+    if (cb != 0.0) s += cb * bv[i];
+    rv[i] = s;
+  }
+}
+
+// fused RK4 stage update (rk.c:231-252): uc += wc*r ; v = up + wv*r
+// (wv == 0: last stage, no v)
+__global__ void rk4_update_kernel(Ptrs25 uc, Ptrs25 up, Ptrs25 r, Ptrs25 v, double wc, double wv, long n) {
+  const int k = blockIdx.y;
+  double *__restrict__ c = uc.p[k];
+  const double *__restrict__ p = up.p[k];
+  const double *__restrict__ rr = r.p[k];
+  double *__restrict__ vv = v.p[k];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double ri = rr[i];
+    c[i] += wc * ri;
+    if (wv != 0.0) vv[i] = p[i] + wv * ri;
+  }
+}
+
+static dim3 stream_grid(long n, int nv) {
+  long blocks = (n + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  return dim3((unsigned)blocks, nv, 1);
+}
+
+// ---------------------------------------------------------------- radiative
+// per-variable far limit and speed (z4_startup, z4.c:200-228)
+struct RadInfo {
+  double var0[25], v[25];
+};
+
+// one thread per surface point of one box, all 25 variables
+__global__ void radiative_kernel(Ptrs25 unew, Ptrs25 upre, Ptrs25 ucur, RadInfo ri, DevBox B, double c) {
+  const int M = B.m, N = B.n, O = B.o;
+  const long nface_x = (long)N * O, nface_y = (long)(M - 2) * O, nface_z = (long)(M - 2) * (N - 2);
+  const long total = 2 * (nface_x + nface_y + nface_z);
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int i, j, k, face;
+  long r = t;
+  if (r < 2 * nface_x) {
+    face = (int)(r / nface_x);
+    r %= nface_x;
+    i = face ? M - 1 : 0;
+    j = (int)(r % N);
+    k = (int)(r / N);
+  } else if ((r -= 2 * nface_x) < 2 * nface_y) {
+    face = 2 + (int)(r / nface_y);
+    r %= nface_y;
+    j = (face == 3) ? N - 1 : 0;
+    i = 1 + (int)(r % (M - 2));
+    k = (int)(r / (M - 2));
+  } else {
+    r -= 2 * nface_y;
+    face = 4 + (int)(r / nface_z);
+    r %= nface_z;
+    k = (face == 5) ? O - 1 : 0;
+    i = 1 + (int)(r % (M - 2));
+    j = 1 + (int)(r / (M - 2));
+  }
+  if (j < 0 || j >= N || k < 0 || k >= O || i < 0 || i >= M) return;
+  // a surface point is PHYBOUND unless a face whose flagged layers contain it
+  // carries another flag (periodic / ghost / symmetry / refinement override)
+  const int dist[6] = {i, M - 1 - i, j, N - 1 - j, k, O - 1 - k};
+  bool phy = false, other = false;
+  for (int f = 0; f < 6; f++) {
+    if (dist[f] == 0 && B.bflag[f] == BAMHIP_PHYBOUND) phy = true;
+    if (dist[f] < B.skip[f] && B.bflag[f] != BAMHIP_PHYBOUND) other = true;
+  }
+  if (!phy || other) return;
+  const long di = 1, dj = M, dk = (long)M * N;
+  const long q = B.noffset + i + dj * j + dk * k;
+  // neighbour indices, a missing neighbour equals q (bam_amr_loops_box.h:338-386)
+  const long mcc = (i > 0) ? q - di : q, pcc = (i + 1 < M) ? q + di : q;
+  const long cmc = (j > 0) ? q - dj : q, cpc = (j + 1 < N) ? q + dj : q;
+  const long ccm = (k > 0) ? q - dk : q, ccp = (k + 1 < O) ? q + dk : q;
+  const long Mcc = (i > 1) ? mcc - di : q, Pcc = (i + 2 < M) ? pcc + di : q;
+  const long cMc = (j > 1) ? cmc - dj : q, cPc = (j + 2 < N) ? cpc + dj : q;
+  const long ccM = (k > 1) ? ccm - dk : q, ccP = (k + 2 < O) ? ccp + dk : q;
+  const double x = B.x0 + i * B.dx, y = B.y0 + j * B.dy, z = B.z0 + k * B.dz;
+  const double rr = sqrt(x * x + y * y + z * z);
+  const double nx = x / rr, ny = y / rr, nz = z / rr;
+  // one-sided or centred 2nd-order difference times 2h (radiative.c:14-40)
+  auto del = [&](const double *u, double vx, long mm, long m, long p, long pp) {
+    const double cen = u[p] - u[m];
+    const double rgt = 4 * u[p] - u[pp] - 3 * u[q];
+    const double lft = 3 * u[q] + u[mm] - 4 * u[m];
+    if (vx > 0) return (mm != q) ? lft : ((m != q) ? cen : rgt);
+    if (vx < 0) return (pp != q) ? rgt : ((m != q) ? cen : lft);
+    if (p != q && m != q) return cen;
+    return (m == q) ? rgt : lft;
+  };
+  for (int v = 0; v < 25; v++) {
+    const double *u = ucur.p[v];
+    const double sp = ri.v[v];
+    const double vx = sp * nx, vy = sp * ny, vz = sp * nz;
+    double rhs = -sp * (u[q] - ri.var0[v]) / rr;
+    rhs -= vx * del(u, vx, Mcc, mcc, pcc, Pcc) / (2 * B.dx) + vy * del(u, vy, cMc, cmc, cpc, cPc) / (2 * B.dy) +
+           vz * del(u, vz, ccM, ccm, ccp, ccP) / (2 * B.dz);
+    unew.p[v][q] = (c != 0.0) ? upre.p[v][q] + c * rhs : rhs;
+  }
+}
+
+// periodic copy of g layers along direction d for nv variables
+// (periodic.c:476-560 with g = 2): layers [0,g) <- [n-2g, n-g), [n-g, n) <- [g, 2g)
+__global__ void periodic_kernel(Ptrs25 u, int nv, DevBox B, int d, int g) {
+  const int dims[3] = {B.m, B.n, B.o};
+  const long st[3] = {1, B.m, (long)B.m * B.n};
+  const int e1 = (d + 1) % 3, e2 = (d + 2) % 3;
+  const long plane = (long)dims[e1] * dims[e2];
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= plane * g) return;
+  const int l = (int)(t / plane);
+  const long r = t % plane;
+  const int a = (int)(r % dims[e1]), b = (int)(r / dims[e1]);
+  const long base = B.noffset + a * st[e1] + b * st[e2];
+  const int nd = dims[d];
+  const int v = blockIdx.y;
+  if (v >= nv) return;
+  double *p = u.p[v];
+  p[base + l * st[d]] = p[base + (nd - 2 * g + l) * st[d]];
+  p[base + (nd - g + l) * st[d]] = p[base + (g + l) * st[d]];
+}
+
+// ---------------------------------------------------------------- algcon
+// z4_algcon.m:14-48 on all points
+__global__ void algcon_kernel(Ptrs25 g6, Ptrs25 A6, long n, int subtractA, int normalizedetg) {
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    double g[6], A[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      g[c] = g6.p[c][q];
+      A[c] = A6.p[c][q];
+    }
+    auto det = [&]() {
+      return g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
+             g[2] * (g[1] * g[4] - g[3] * g[2]);
+    };
+    double detg = det();
+    if (detg <= 0.0) detg = 1.0;
+    if (normalizedetg) {
+      const double e = detg - 1.0;
+      const double aux = (fabs(e) < 1e-12) ? 1.0 - e / 3.0 : pow(detg, -1.0 / 3.0);
+#pragma unroll
+      for (int c = 0; c < 6; c++) g[c] *= aux;
+    }
+    detg = det();
+    if (detg <= 0.0) detg = 1.0;
+    const double a0 = g[3] * g[5] - g[4] * g[4], a1 = g[2] * g[4] - g[1] * g[5], a2 = g[1] * g[4] - g[2] * g[3];
+    const double a3 = g[0] * g[5] - g[2] * g[2], a4 = g[1] * g[2] - g[0] * g[4], a5 = g[0] * g[3] - g[1] * g[1];
+    const double trA = (a0 * A[0] + a3 * A[3] + a5 * A[5] + 2.0 * (a1 * A[1] + a2 * A[2] + a4 * A[4])) / detg;
+    if (subtractA) {
+#pragma unroll
+      for (int c = 0; c < 6; c++) A[c] -= trA / 3.0 * g[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      g6.p[c][q] = g[c];
+      A6.p[c][q] = A[c];
+    }
+  }
+}
+
+// -------------------------------------------------- Hamiltonian constraint
+// z4_to_adm (z4.c:93-124) into 12 scratch fields, then the order-4 (order 2
+// one point from the edge) Hamiltonian constraint of adm_constraints_N.m at
+// inner points, then a per-block partial sum of H^2 and a count.
+__global__ void z4_to_adm_kernel(Ptrs25 u, double *adm, long n, double chipsipower) {
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const double psi = pow(u.p[6][q], 1.0 / chipsipower);
+    const double psi4 = pow(psi, 4.0);
+    const double Kh = u.p[13][q];
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      const double gg = psi4 * u.p[c][q];
+      adm[c * n + q] = gg;
+      adm[(6 + c) * n + q] = psi4 * u.p[7 + c][q] + Kh * gg / 3.0;
+    }
+  }
+}
+
+__device__ __forceinline__ int syi(int a, int b) {
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  return lo * (5 - lo) / 2 + hi;
+}
+
+__global__ void __launch_bounds__(256) ham_kernel(const double *__restrict__ adm, long n, DevBox B, double *ham,
+                                                  double *partial) {
+  __shared__ double red[256];
+  __shared__ double cnt[256];
+  const int M = B.m, N = B.n, O = B.o;
+  const int i = B.skip[0] + blockIdx.x * 64 + threadIdx.x;
+  const int j = B.skip[2] + blockIdx.y * 4 + threadIdx.y;
+  const int k = B.skip[4] + blockIdx.z;
+  const int tid = threadIdx.x + 64 * threadIdx.y;
+  double h2 = 0.0, c = 0.0;
+  if (i <= M - 1 - B.skip[1] && j <= N - 1 - B.skip[3] && k <= O - 1 - B.skip[5]) {
+    const long st[3] = {1, M, (long)M * N};
+    const double h[3] = {B.dx, B.dy, B.dz};
+    const long q = B.noffset + i + st[1] * j + st[2] * k;
+    const bool o2 = (i == 1 || j == 1 || k == 1 || i == M - 2 || j == N - 2 || k == O - 2);
+    double g[6], Kd[6], dg[3][6], ddg[6][6];
+    for (int c6 = 0; c6 < 6; c6++) {
+      const double *f = adm + c6 * n;
+      g[c6] = f[q];
+      Kd[c6] = adm[(6 + c6) * n + q];
+      for (int e = 0; e < 3; e++) {
+        const long s = st[e];
+        dg[e][c6] = o2 ? (f[q + s] - f[q - s]) / (2 * h[e])
+                       : (f[q - 2 * s] - 8 * f[q - s] + 8 * f[q + s] - f[q + 2 * s]) / (12 * h[e]);
+      }
+      for (int e = 0; e < 3; e++)
+        for (int e2 = e; e2 < 3; e2++) {
+          const long s = st[e], s2 = st[e2];
+          double v;
+          if (e == e2)
+            v = o2 ? (f[q - s] - 2 * f[q] + f[q + s]) / (h[e] * h[e])
+                   : (-f[q - 2 * s] + 16 * f[q - s] - 30 * f[q] + 16 * f[q + s] - f[q + 2 * s]) / (12 * h[e] * h[e]);
+          else if (o2)
+            v = (f[q + s + s2] - f[q + s - s2] - f[q - s + s2] + f[q - s - s2]) / (4 * h[e] * h[e2]);
+          else {
+            const int of[4] = {-2, -1, 1, 2};
+            const double w[4] = {1, -8, 8, -1};
+            double t = 0;
+            for (int a = 0; a < 4; a++)
+              for (int b2 = 0; b2 < 4; b2++) t += w[a] * w[b2] * f[q + of[a] * s + of[b2] * s2];
+            v = t / (144 * h[e] * h[e2]);
+          }
+          ddg[syi(e, e2)][c6] = v;
+        }
+    }
+    double gi[6];
+    gi[0] = g[3] * g[5] - g[4] * g[4];
+    gi[1] = g[2] * g[4] - g[1] * g[5];
+    gi[2] = g[1] * g[4] - g[2] * g[3];
+    gi[3] = g[0] * g[5] - g[2] * g[2];
+    gi[4] = g[1] * g[2] - g[0] * g[4];
+    gi[5] = g[0] * g[3] - g[1] * g[1];
+    const double id = 1.0 / (g[0] * gi[0] + g[1] * gi[1] + g[2] * gi[2]);
+    for (int c6 = 0; c6 < 6; c6++) gi[c6] *= id;
+    double gdo[3][6], gam[3][6];
+    for (int cc = 0; cc < 3; cc++)
+      for (int a = 0; a < 3; a++)
+        for (int b = a; b < 3; b++)
+          gdo[cc][syi(a, b)] = 0.5 * (dg[a][syi(b, cc)] + dg[b][syi(a, cc)] - dg[cc][syi(a, b)]);
+    for (int cc = 0; cc < 3; cc++)
+      for (int ab = 0; ab < 6; ab++)
+        gam[cc][ab] = gi[syi(cc, 0)] * gdo[0][ab] + gi[syi(cc, 1)] * gdo[1][ab] + gi[syi(cc, 2)] * gdo[2][ab];
+    double R = 0.0;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        double Rab = 0.0;
+        for (int cc = 0; cc < 3; cc++)
+          for (int d = 0; d < 3; d++) {
+            double t = 0.5 * (-ddg[syi(cc, d)][syi(a, b)] - ddg[syi(a, b)][syi(cc, d)] + ddg[syi(a, cc)][syi(b, d)] +
+                              ddg[syi(b, cc)][syi(a, d)]);
+            for (int e = 0; e < 3; e++) t += gam[e][syi(a, cc)] * gdo[e][syi(b, d)] - gam[e][syi(a, b)] * gdo[e][syi(cc, d)];
+            Rab += gi[syi(cc, d)] * t;
+          }
+        R += gi[syi(a, b)] * Rab;
+      }
+    double Kud[3][3];
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++)
+        Kud[a][b] = gi[syi(a, 0)] * Kd[syi(0, b)] + gi[syi(a, 1)] * Kd[syi(1, b)] + gi[syi(a, 2)] * Kd[syi(2, b)];
+    double trK = Kud[0][0] + Kud[1][1] + Kud[2][2], KK = 0.0;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) KK += Kud[a][b] * Kud[b][a];
+    const double H = R + trK * trK - KK;
+    if (ham) ham[q] = H;
+    h2 = H * H;
+    c = 1.0;
+  }
+  red[tid] = h2;
+  cnt[tid] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      red[tid] += red[tid + s];
+      cnt[tid] += cnt[tid + s];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const long bid = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    partial[2 * bid] = red[0];
+    partial[2 * bid + 1] = cnt[0];
+  }
+}
+
+static int get25(bamhip_level *lev, const int *idx, int n, Ptrs25 &P, const char *what) {
+  for (int v = 0; v < 25; v++) P.p[v] = nullptr;
+  return gather_ptrs(lev, idx, n, P.p, what);
+}
+
+}  // namespace bamhip
+
+using namespace bamhip;
+
+extern "C" {
+
+int bamhip_level_fill(bamhip_level *lev, int var, double value) {
+  double *p = bamhip_level_ptr(lev, var);
+  if (!p) return set_error(BAMHIP_EINVAL, "bamhip_level_fill: variable %d not enabled", var);
+  fill_kernel<<<stream_grid(lev->npoints, 1).x, 256, 0, stream()>>>(p, lev->npoints, value);
+  BH_HIP(hipGetLastError());
+  return BAMHIP_OK;
+}
+
+int bamhip_vladd(bamhip_level *lev, int n, const int *r, double ca, const int *a, double cb, const int *b) {
+  if (!lev || n < 1 || n > 25) return set_error(BAMHIP_EINVAL, "bamhip_vladd: n=%d (1..25)", n);
+  Ptrs25 R, A, Bp;
+  int e;
+  if ((e = get25(lev, r, n, R, "bamhip_vladd(r)"))) return e;
+  if (ca != 0.0 && (e = get25(lev, a, n, A, "bamhip_vladd(a)"))) return e;
+  if (cb != 0.0 && (e = get25(lev, b, n, Bp, "bamhip_vladd(b)"))) return e;
+  if (ca == 0.0) A = R;
+  if (cb == 0.0) Bp = R;
+  TimerScope t("vladd");
+  vladd_kernel<<<stream_grid(lev->npoints, n), 256, 0, stream()>>>(R, A, Bp, ca, cb, lev->npoints);
+  BH_HIP(hipGetLastError());
+  return BAMHIP_OK;
+}
+
+int bamhip_z4c_set_boundary(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, const int *upre,
+                            double c, const int *ucur, int pgh) {
+  if (!lev || !par) return set_error(BAMHIP_EINVAL, "bamhip_z4c_set_boundary: NULL argument");
+  Ptrs25 N, Pp, U;
+  int e;
+  if ((e = get25(lev, unew, 25, N, "bamhip_z4c_set_boundary(unew)"))) return e;
+  if ((e = get25(lev, ucur, 25, U, "bamhip_z4c_set_boundary(ucur)"))) return e;
+  if (c != 0.0) {
+    if ((e = get25(lev, upre, 25, Pp, "bamhip_z4c_set_boundary(upre)"))) return e;
+  } else {
+    Pp = U;
+  }
+  RadInfo ri;
+  double vg = 1.0;
+  if (par->oploglapse) vg = sqrt(par->lapseharmonicf);
+  else if (par->oploglapse2) vg = sqrt(4.0 / 3.0);
+  for (int v = 0; v < 25; v++) {
+    ri.var0[v] = (v == 0 || v == 3 || v == 5 || v == 6 || v == 18) ? 1.0 : 0.0;
+    ri.v[v] = (v == 6 || v == 18 || v == 13) ? vg : 1.0;
+  }
+  TimerScope t("set_boundary");
+  for (const DevBox &B : lev->boxes) {
+    bool anyphy = false, anyper = false;
+    for (int f = 0; f < 6; f++) {
+      anyphy |= B.bflag[f] == BAMHIP_PHYBOUND;
+      anyper |= B.bflag[f] == BAMHIP_PERBOUND;
+    }
+    if (anyphy) {
+      if (B.m < 3 || B.n < 3 || B.o < 3)
+        return set_error(BAMHIP_EINVAL, "bamhip_z4c_set_boundary: box too small for the radiative condition");
+      const long total = 2L * ((long)B.n * B.o + (long)(B.m - 2) * B.o + (long)(B.m - 2) * (B.n - 2));
+      radiative_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream()>>>(N, Pp, U, ri, B, c);
+      BH_HIP(hipGetLastError());
+    }
+    if (anyper) {
+      if (pgh < 1) return set_error(BAMHIP_EINVAL, "bamhip_z4c_set_boundary: periodic faces need pgh >= 1");
+      const int dims[3] = {B.m, B.n, B.o};
+      for (int d = 0; d < 3; d++) {
+        if (B.bflag[2 * d] != BAMHIP_PERBOUND || B.bflag[2 * d + 1] != BAMHIP_PERBOUND) continue;
+        if (dims[d] < 3 * pgh)
+          return set_error(BAMHIP_EINVAL, "bamhip_z4c_set_boundary: periodic direction %d has %d < 3*%d points", d,
+                           dims[d], pgh);
+        const long plane = (long)dims[(d + 1) % 3] * dims[(d + 2) % 3];
+        dim3 grid((unsigned)((plane * pgh + 255) / 256), 25, 1);
+        periodic_kernel<<<grid, 256, 0, stream()>>>(N, 25, B, d, pgh);
+        BH_HIP(hipGetLastError());
+      }
+    }
+  }
+  return BAMHIP_OK;
+}
+
+int bamhip_z4c_algcon(bamhip_level *lev, const int *g, const int *A, int subtractA, int normalizedetg) {
+  if (!lev) return set_error(BAMHIP_EINVAL, "bamhip_z4c_algcon: NULL level");
+  Ptrs25 Gp, Ap;
+  int e;
+  if ((e = get25(lev, g, 6, Gp, "bamhip_z4c_algcon(g)"))) return e;
+  if ((e = get25(lev, A, 6, Ap, "bamhip_z4c_algcon(A)"))) return e;
+  TimerScope t("algcon");
+  algcon_kernel<<<stream_grid(lev->npoints, 1).x, 256, 0, stream()>>>(Gp, Ap, lev->npoints, subtractA, normalizedetg);
+  BH_HIP(hipGetLastError());
+  return BAMHIP_OK;
+}
+
+int bamhip_rk4_stage_update(bamhip_level *lev, int n, const int *uc, const int *up, const int *r, const int *v,
+                            double wc, double wv) {
+  Ptrs25 C, P, R, V;
+  int e;
+  if ((e = get25(lev, uc, n, C, "rk4_update(uc)"))) return e;
+  if ((e = get25(lev, r, n, R, "rk4_update(r)"))) return e;
+  if (wv != 0.0) {
+    if ((e = get25(lev, up, n, P, "rk4_update(up)"))) return e;
+    if ((e = get25(lev, v, n, V, "rk4_update(v)"))) return e;
+  } else {
+    P = C;
+    V = C;
+  }
+  TimerScope t("rk4_update");
+  rk4_update_kernel<<<stream_grid(lev->npoints, n), 256, 0, stream()>>>(C, P, R, V, wc, wv, lev->npoints);
+  BH_HIP(hipGetLastError());
+  return BAMHIP_OK;
+}
+
+int bamhip_z4c_ham_l2(bamhip_level *lev, const int *u, double chipsipower, int ham, double *l2) {
+  if (!lev || !l2) return set_error(BAMHIP_EINVAL, "bamhip_z4c_ham_l2: NULL argument");
+  Ptrs25 U;
+  int e;
+  if ((e = get25(lev, u, 25, U, "bamhip_z4c_ham_l2(u)"))) return e;
+  double *hp = nullptr;
+  if (ham >= 0) {
+    hp = bamhip_level_ptr(lev, ham);
+    if (!hp) return set_error(BAMHIP_EINVAL, "bamhip_z4c_ham_l2: ham variable %d not enabled", ham);
+  }
+  double *adm = nullptr;
+  BH_HIP(hipMallocAsync((void **)&adm, sizeof(double) * 12 * lev->npoints, stream()));
+  z4_to_adm_kernel<<<stream_grid(lev->npoints, 1).x, 256, 0, stream()>>>(U, adm, lev->npoints, chipsipower);
+  double sum = 0.0, count = 0.0;
+  for (const DevBox &B : lev->boxes) {
+    const int ni = B.m - B.skip[0] - B.skip[1], nj = B.n - B.skip[2] - B.skip[3], nk = B.o - B.skip[4] - B.skip[5];
+    if (ni <= 0 || nj <= 0 || nk <= 0) continue;
+    dim3 grid((ni + 63) / 64, (nj + 3) / 4, nk);
+    const long nb = (long)grid.x * grid.y * grid.z;
+    double *partial = nullptr;
+    BH_HIP(hipMallocAsync((void **)&partial, sizeof(double) * 2 * nb, stream()));
+    ham_kernel<<<grid, dim3(64, 4, 1), 0, stream()>>>(adm, lev->npoints, B, hp, partial);
+    std::vector<double> hpart(2 * nb);
+    BH_HIP(hipMemcpyAsync(hpart.data(), partial, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, stream()));
+    BH_HIP(hipStreamSynchronize(stream()));
+    BH_HIP(hipFreeAsync(partial, stream()));
+    for (long b = 0; b < nb; b++) {
+      sum += hpart[2 * b];
+      count += hpart[2 * b + 1];
+    }
+  }
+  BH_HIP(hipFreeAsync(adm, stream()));
+  BH_HIP(hipStreamSynchronize(stream()));
+  *l2 = count > 0 ? sqrt(sum / count) : 0.0;
+  return BAMHIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1168 @@
+// z4c_rhs.hip -- Z4c right-hand side on gfx950, FP64.
+//
+// Equations: src/projects/z4/z4_rhs_movpunc_N.m:217-467 (conformal
+// connection and Ricci, conformal-factor terms, 1+log / Gamma-driver gauge,
+// Kreiss-Oliger dissipation, optional addlinear update with trace removal and
+// det normalisation).  Stencils: src/math/MathToC/discretize.m:18-46,97-135
+// and src/main/amr/bam_amr_advection.h:28-155.  Box-edge rules (order 2 one
+// point from the edge, shifted advection stencils within rstencil points, KO
+// only two or more points away): src/main/amr/bam_amr_loops_box.h:186-215.
+//
+// Layout: SoA, one double array per grid function, ijk = i + m*j + m*n*k
+// within a box, boxes concatenated at noffset.  One thread per grid point;
+// a block is a 64 x 4 x 1 tile (one wave per x-row of 64 points, so every
+// x-neighbour load is a coalesced 512 B transaction and y/z neighbours are
+// served from L1/L2).  The algebra is ordered so that derivative groups are
+// consumed as soon as they are formed (second derivatives of gt go straight
+// into the Ricci accumulators, those of beta into the Gamma-driver terms),
+// keeping the live FP64 state within the VGPR budget of one wave per SIMD.
+//
+// Two derivative providers share the algebra:
+//   Bulk4 : points >= 3 away from every box edge, 4th-order centred
+//           stencils, lop-sided (offset 1) 4th-order advection;
+//   Edge  : any point, with the reference's per-point order selection,
+//           used for the shell of points 1 and 2 away from the box edge and
+//           for parameter choices other than (4, 4, lop-sided 1).
+#include <cmath>
+
+#include "bamhip_internal.h"
+#include <utility>
+
+// compile-time loop: f(integral_constant<int, I>) for I = 0..N-1; used for
+// loops whose body holds a barrier, which the compiler does not unroll, so
+// that per-field arrays keep constant indices and stay in registers
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+namespace bamhip {
+
+struct Z4cPtrs {
+  const double *u[BAMHIP_Z4C_NVARS];
+  double *r[BAMHIP_Z4C_NVARS];
+  const double *p[BAMHIP_Z4C_NVARS];
+};
+
+struct Z4cGeo {
+  int m, n, o;      // box points
+  long noffset;
+  int lo[3], hi[3]; // computed index range [lo, hi] per direction
+  long sj, sk;      // strides (di = 1)
+  double h[3], ih[3];
+  double i12h[3], i12hh[3], i144hh[3];  // 4th order: 1/(12h), 1/(12h^2), mixed 1/(144 ha hb) (xy, xz, yz)
+  double i2h[3], ihh[3], i4hh[3];       // 2nd order: 1/(2h), 1/h^2, mixed 1/(4 ha hb)
+  int edge_only;    // Edge kernel: skip points that the bulk kernel computes
+};
+
+struct Z4cK {
+  int order_centered, order_advection, lopsided, order_dissipation;
+  double dissfactor, kappa1, kappa2, zeroThetaRHS;
+  int subtractA, normalizedetg;
+  double oploglapse, oploglapse2, oplogwithshift, harmoniclapse, lapseharmonicf;
+  double gamma0factor, gamma2factor, withGadv, withShiftadv, withBadv, withB;
+  double shiftalphapower, shiftgammacoeff, shiftdriver;
+  double chiDivFloor, chipsipower;
+  double c;
+  int addlinear;
+};
+
+// scheduling fence: keeps the machine scheduler from hoisting the loads of
+// later derivative groups above earlier ones, which would keep hundreds of
+// FP64 stencil values live at once and spill
+#define BH_PHASE() __builtin_amdgcn_sched_barrier(0)
+
+// index of component (a,b) in xx xy xz yy yz zz storage; closed form (not
+// recursive) so that it folds to a constant once the loops are unrolled and
+// every tensor array stays in registers
+__device__ __forceinline__ constexpr int sy(int a, int b) {
+  return (a < b ? a : b) * (5 - (a < b ? a : b)) / 2 + (a < b ? b : a);
+}
+__device__ __forceinline__ constexpr int mixid(int a, int b) {  // xy=0 xz=1 yz=2
+  return (a + b) - 1;
+}
+
+// ---------------------------------------------------------------- providers
+// runtime-order provider following the reference per point
+struct Edge {
+  long q;
+  long s[3];
+  const Z4cGeo &G;
+  int ordc, orda, twoormore;
+  int pos[3], pmax[3];
+  __device__ Edge(const Z4cGeo &g, long q_, int i, int j, int k, const Z4cK &K) : q(q_), G(g) {
+    s[0] = 1;
+    s[1] = g.sj;
+    s[2] = g.sk;
+    pos[0] = i;
+    pos[1] = j;
+    pos[2] = k;
+    pmax[0] = g.m - 1;
+    pmax[1] = g.n - 1;
+    pmax[2] = g.o - 1;
+    const int away1 = (i == 1 || j == 1 || k == 1 || i == pmax[0] - 1 || j == pmax[1] - 1 || k == pmax[2] - 1);
+    twoormore = (i >= 2 && j >= 2 && k >= 2 && i <= pmax[0] - 2 && j <= pmax[1] - 2 && k <= pmax[2] - 2);
+    ordc = (K.order_centered == 2 || away1) ? 2 : 4;
+    orda = (K.order_advection == 2 || away1) ? 2 : 4;
+  }
+  __device__ double d1(const double *__restrict__ f, int a) const {
+    const long t = s[a];
+    if (ordc == 2) return (f[q + t] - f[q - t]) * G.i2h[a];
+    return ((f[q - 2 * t] - f[q + 2 * t]) + 8.0 * (f[q + t] - f[q - t])) * G.i12h[a];
+  }
+  __device__ double d2(const double *__restrict__ f, int a) const {
+    const long t = s[a];
+    if (ordc == 2) return (f[q - t] + f[q + t] - 2.0 * f[q]) * G.ihh[a];
+    return (16.0 * (f[q - t] + f[q + t]) - (f[q - 2 * t] + f[q + 2 * t]) - 30.0 * f[q]) * G.i12hh[a];
+  }
+  __device__ double dd(const double *__restrict__ f, int a, int b) const {
+    if (a == b) return d2(f, a);
+    const long t = s[a], w = s[b];
+    if (ordc == 2) {
+      return ((f[q + t + w] - f[q + t - w]) - (f[q - t + w] - f[q - t - w])) * G.i4hh[mixid(a, b)];
+    }
+    auto row = [&](long o) { return (f[o - 2 * w] - f[o + 2 * w]) + 8.0 * (f[o + w] - f[o - w]); };
+    return ((row(q - 2 * t) - row(q + 2 * t)) + 8.0 * (row(q + t) - row(q - t))) * G.i144hh[mixid(a, b)];
+  }
+  // advection weights over offsets -4..4 (slot 4 = centre)
+  double W[3][9];
+  __device__ void set_adv(const double beta[3], const Z4cK &K) {
+    const double st4[5][5] = {{25. / 12, -48. / 12, 36. / 12, -16. / 12, 3. / 12},
+                              {3. / 12, 10. / 12, -18. / 12, 6. / 12, -1. / 12},
+                              {-1. / 12, 8. / 12, 0. / 12, -8. / 12, 1. / 12},
+                              {1. / 12, -6. / 12, 18. / 12, -10. / 12, -3. / 12},
+                              {-3. / 12, 16. / 12, -36. / 12, 48. / 12, -25. / 12}};
+    const double st2[3][3] = {{1.5, -2.0, 0.5}, {0.5, 0.0, -0.5}, {-0.5, 2.0, -1.5}};
+    for (int a = 0; a < 3; a++) {
+      for (int k = 0; k < 9; k++) W[a][k] = 0.0;
+      const double wx = beta[a] * G.ih[a];
+      const int i = pos[a], imax = pmax[a];
+      const int rst = (orda == 2) ? 2 : 4 - K.lopsided;
+      int wsign, offset;
+      if (wx < 0) {
+        wsign = -1;
+        offset = (i >= rst) ? 0 : rst - i;
+      } else if (wx > 0) {
+        wsign = 1;
+        offset = (imax - i >= rst) ? 0 : rst - (imax - i);
+      } else {
+        wsign = 0;
+        offset = 0;
+      }
+      if (wsign == 0) continue;
+      if (orda == 2) {
+        for (int k = 0; k < 3; k++) W[a][4 + wsign * (k - offset)] += -wsign * wx * st2[offset][k];
+      } else {
+        offset += K.lopsided;
+        for (int k = 0; k < 5; k++) W[a][4 + wsign * (k - offset)] += -wsign * wx * st4[offset][k];
+      }
+    }
+  }
+  __device__ double adv(const double *__restrict__ f) const {
+    double acc = 0;
+    for (int a = 0; a < 3; a++)
+      for (int k = 0; k < 9; k++)
+        if (W[a][k] != 0.0) acc += W[a][k] * f[q + (k - 4) * s[a]];
+    return acc;
+  }
+  __device__ bool ko_on(const Z4cK &K) const { return K.order_dissipation == 4 && twoormore; }
+  __device__ double ko(const double *__restrict__ f) const {
+    double acc = 0;
+    for (int a = 0; a < 3; a++) {
+      const long t = s[a];
+      acc += ((f[q - 2 * t] + f[q + 2 * t]) - 4.0 * (f[q - t] + f[q + t]) + 6.0 * f[q]) * G.ih[a];
+    }
+    return -acc;
+  }
+};
+
+// write one point: raw RHS, or upre + c*RHS with trace removal and det
+// normalisation (z4_rhs_movpunc_N.m:446-467)
+__device__ __forceinline__ void store_point(const Z4cPtrs &P, const Z4cK &K, long q, double *rhs) {
+  if (K.addlinear) {
+#pragma unroll
+    for (int v = 0; v < 25; v++) rhs[v] = P.p[v][q] + K.c * rhs[v];
+    double ng[6], nA[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      ng[c] = rhs[c];
+      nA[c] = rhs[7 + c];
+    }
+    const double a0 = ng[3] * ng[5] - ng[4] * ng[4], a1 = ng[2] * ng[4] - ng[1] * ng[5];
+    const double a2 = ng[1] * ng[4] - ng[2] * ng[3], a3 = ng[0] * ng[5] - ng[2] * ng[2];
+    const double a4 = ng[1] * ng[2] - ng[0] * ng[4], a5 = ng[0] * ng[3] - ng[1] * ng[1];
+    const double detnginv = 1.0 / (ng[0] * a0 + ng[1] * a1 + ng[2] * a2);
+    if (K.subtractA) {
+      const double trA =
+          detnginv * (a0 * nA[0] + a3 * nA[3] + a5 * nA[5] + 2.0 * (a1 * nA[1] + a2 * nA[2] + a4 * nA[4]));
+#pragma unroll
+      for (int c = 0; c < 6; c++) rhs[7 + c] = nA[c] - (trA / 3.0) * ng[c];
+    }
+    if (K.normalizedetg) {
+      const double aux = pow(detnginv, 1.0 / 3.0);
+#pragma unroll
+      for (int c = 0; c < 6; c++) rhs[c] = aux * ng[c];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 25; v++) P.r[v][q] = rhs[v];
+}
+
+// ------------------------------------------------------------------ algebra
+template <class D>
+__device__ __forceinline__ bool z4c_point(D &dv, const Z4cPtrs &P, const Z4cK &K, long q) {
+  const double *const *u = P.u;
+  // point values (list order of z4.c:234-242)
+  double g[6], A[6], Gm[3], beta[3], Bv[3];
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    g[c] = u[c][q];
+    A[c] = u[7 + c][q];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    Gm[a] = u[14 + a][q];
+    beta[a] = u[19 + a][q];
+    Bv[a] = u[22 + a][q];
+  }
+  const double chi = u[6][q], Khat = u[13][q], Theta = u[17][q], alpha = u[18][q];
+
+  // advection terms (lie*, adv*) of all 25 fields, weighted by their
+  // coefficients in the right-hand sides
+  dv.set_adv(beta, K);
+  double rhs[25];
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    rhs[c] = dv.adv(u[c]);
+    rhs[7 + c] = dv.adv(u[7 + c]);
+  }
+  rhs[6] = dv.adv(u[6]);
+  rhs[13] = dv.adv(u[13]);
+  double advG[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    advG[a] = dv.adv(u[14 + a]);
+    rhs[19 + a] = K.withShiftadv * dv.adv(u[19 + a]);
+    rhs[22 + a] = K.withBadv * dv.adv(u[22 + a]);
+  }
+  rhs[17] = dv.adv(u[17]);
+  rhs[18] = K.oplogwithshift * dv.adv(u[18]);
+
+  BH_PHASE();
+  // inverse conformal metric
+  double ig[6];
+  ig[0] = g[3] * g[5] - g[4] * g[4];
+  ig[1] = g[2] * g[4] - g[1] * g[5];
+  ig[2] = g[1] * g[4] - g[2] * g[3];
+  ig[3] = g[0] * g[5] - g[2] * g[2];
+  ig[4] = g[1] * g[2] - g[0] * g[4];
+  ig[5] = g[0] * g[3] - g[1] * g[1];
+  const double detginv = 1.0 / (g[0] * ig[0] + g[1] * ig[1] + g[2] * ig[2]);
+#pragma unroll
+  for (int c = 0; c < 6; c++) ig[c] *= detginv;
+
+  BH_PHASE();
+  // Christoffels: lower Gd[c][ab] = Gamma_{c ab}, upper Gu[c][ab] = Gamma^c_ab
+  double Gd[3][6];
+  {
+    double dg[3][6];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int ab = 0; ab < 6; ab++) dg[c][ab] = dv.d1(u[ab], c);
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = a; b < 3; b++)
+          Gd[c][sy(a, b)] = 0.5 * (dg[a][sy(b, c)] + dg[b][sy(a, c)] - dg[c][sy(a, b)]);
+  }
+  double Gu[3][6];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int ab = 0; ab < 6; ab++)
+      Gu[c][ab] = ig[sy(c, 0)] * Gd[0][ab] + ig[sy(c, 1)] * Gd[1][ab] + ig[sy(c, 2)] * Gd[2][ab];
+  double Gfg[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+    Gfg[a] = ig[0] * Gu[a][0] + ig[3] * Gu[a][3] + ig[5] * Gu[a][5] +
+             2.0 * (ig[1] * Gu[a][1] + ig[2] * Gu[a][2] + ig[4] * Gu[a][4]);
+
+  BH_PHASE();
+  // Ricci of the conformal metric
+  double R[6];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = a; b < 3; b++) {
+      double s = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          double t = 0.0;
+#pragma unroll
+          for (int e = 0; e < 3; e++)
+            t += Gu[e][sy(c, a)] * Gd[b][sy(e, d)] + Gu[e][sy(c, b)] * Gd[a][sy(e, d)] +
+                 Gu[e][sy(a, d)] * Gd[e][sy(c, b)];
+          s += ig[sy(c, d)] * t;
+        }
+      double t2 = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; c++) t2 += Gfg[c] * (Gd[a][sy(b, c)] + Gd[b][sy(a, c)]);
+      R[sy(a, b)] = s + 0.5 * t2;
+    }
+  BH_PHASE();
+  // - 1/2 g^cd d_c d_d g_ab, consumed component by component
+#pragma unroll
+  for (int ab = 0; ab < 6; ab++) {
+    const double *f = u[ab];
+    const double lap = ig[0] * dv.d2(f, 0) + ig[3] * dv.d2(f, 1) + ig[5] * dv.d2(f, 2) +
+                       2.0 * (ig[1] * dv.dd(f, 0, 1) + ig[2] * dv.dd(f, 0, 2) + ig[4] * dv.dd(f, 1, 2));
+    R[ab] -= 0.5 * lap;
+    BH_PHASE();
+  }
+  BH_PHASE();
+  // + 1/2 (g_ca d_b G^c + g_cb d_a G^c)
+  {
+    double dG[3][3];  // dG[b][c] = d_b G^c
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) dG[b][c] = dv.d1(u[14 + c], b);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = a; b < 3; b++) {
+        double t = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) t += g[sy(c, a)] * dG[b][c] + g[sy(c, b)] * dG[a][c];
+        R[sy(a, b)] += 0.5 * t;
+      }
+  }
+
+  BH_PHASE();
+  // conformal factor
+  const double chig = (chi < K.chiDivFloor) ? K.chiDivFloor : chi;
+  const double oocp = 1.0 / K.chipsipower;
+  const double psim4 = exp(-4.0 * oocp * log(chig));
+  const double oochig = 1.0 / chig;
+  double dchi[3], df[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    dchi[a] = dv.d1(u[6], a);
+    df[a] = oocp * dchi[a] * oochig;
+  }
+  double cddf[6];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = a; b < 3; b++) {
+      const int ab = sy(a, b);
+      cddf[ab] = oocp * dv.dd(u[6], a, b) * oochig - K.chipsipower * df[a] * df[b] -
+                 (Gu[0][ab] * df[0] + Gu[1][ab] * df[1] + Gu[2][ab] * df[2]);
+    }
+  auto trace = [&](const double *t6) {
+    return ig[0] * t6[0] + ig[3] * t6[3] + ig[5] * t6[5] + 2.0 * (ig[1] * t6[1] + ig[2] * t6[2] + ig[4] * t6[4]);
+  };
+  auto contract = [&](const double *x, const double *y) {
+    return ig[0] * x[0] * y[0] + ig[3] * x[1] * y[1] + ig[5] * x[2] * y[2] +
+           ig[1] * (x[0] * y[1] + x[1] * y[0]) + ig[2] * (x[0] * y[2] + x[2] * y[0]) +
+           ig[4] * (x[1] * y[2] + x[2] * y[1]);
+  };
+  const double trcddf = trace(cddf);
+  const double dfdf = contract(df, df);
+  // R + Rphi
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = a; b < 3; b++) {
+      const int ab = sy(a, b);
+      R[ab] += 4.0 * df[a] * df[b] - 2.0 * cddf[ab] - 2.0 * g[ab] * trcddf - 4.0 * g[ab] * dfdf;
+    }
+
+  BH_PHASE();
+  // lapse derivatives
+  double da[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) da[a] = dv.d1(u[18], a);
+  const double dfda = contract(df, da);
+  double cdda[6];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = a; b < 3; b++) {
+      const int ab = sy(a, b);
+      cdda[ab] = dv.dd(u[18], a, b) - (Gu[0][ab] * da[0] + Gu[1][ab] * da[1] + Gu[2][ab] * da[2]) -
+                 2.0 * (df[a] * da[b] + df[b] * da[a] - g[ab] * dfda);
+    }
+  const double trcdda = psim4 * trace(cdda);
+
+  BH_PHASE();
+  // extrinsic-curvature terms
+  double Ainv[6], AA[6];
+  {
+    double Au[3][3];  // Au[a][b] = g^ac A_cb (mixed)
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++)
+        Au[a][b] = ig[sy(a, 0)] * A[sy(0, b)] + ig[sy(a, 1)] * A[sy(1, b)] + ig[sy(a, 2)] * A[sy(2, b)];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = a; b < 3; b++) {
+        Ainv[sy(a, b)] = Au[a][0] * ig[sy(0, b)] + Au[a][1] * ig[sy(1, b)] + Au[a][2] * ig[sy(2, b)];
+        AA[sy(a, b)] = A[sy(a, 0)] * Au[0][b] + A[sy(a, 1)] * Au[1][b] + A[sy(a, 2)] * Au[2][b];
+      }
+  }
+  const double cAA = trace(AA);
+  const double K_ = Khat + 2.0 * Theta;
+  double dKT[3];  // 2 dKhat + dTheta
+#pragma unroll
+  for (int a = 0; a < 3; a++) dKT[a] = 2.0 * dv.d1(u[13], a) + dv.d1(u[17], a);
+  double divAinv[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    double s = 0.0;
+#pragma unroll
+    for (int bc = 0; bc < 6; bc++) s += ((bc == 0 || bc == 3 || bc == 5) ? 1.0 : 2.0) * Gu[a][bc] * Ainv[bc];
+    s -= 1.5 * oochig * (Ainv[sy(a, 0)] * dchi[0] + Ainv[sy(a, 1)] * dchi[1] + Ainv[sy(a, 2)] * dchi[2]);
+    s -= (1.0 / 3.0) * (ig[sy(a, 0)] * dKT[0] + ig[sy(a, 1)] * dKT[1] + ig[sy(a, 2)] * dKT[2]);
+    divAinv[a] = s;
+  }
+  const double Rhat = psim4 * trace(R);   // R holds R + Rphi
+  const double Hhat = Rhat + 2.0 * K_ * K_ / 3.0 - cAA;
+
+  BH_PHASE();
+  // shift terms
+  double db[3][3];  // db[a][b] = d_a beta^b
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) db[a][b] = dv.d1(u[19 + b], a);
+  const double divbeta = db[0][0] + db[1][1] + db[2][2];
+  const double tdb = (2.0 / 3.0) * divbeta;
+  double pG[3];
+  {
+    // g^bc d_b d_c beta^a + g^ab (1/3) d_b d_c beta^c
+    double ddbc[3][6];
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++)
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = a; b < 3; b++) ddbc[cc][sy(a, b)] = dv.dd(u[19 + cc], a, b);
+    double ootd[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) ootd[a] = (1.0 / 3.0) * (ddbc[0][sy(a, 0)] + ddbc[1][sy(a, 1)] + ddbc[2][sy(a, 2)]);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+      pG[a] = trace(ddbc[a]) + ig[sy(a, 0)] * ootd[0] + ig[sy(a, 1)] * ootd[1] + ig[sy(a, 2)] * ootd[2] -
+              (Gfg[0] * db[0][a] + Gfg[1] * db[1][a] + Gfg[2] * db[2][a]) + Gfg[a] * tdb + advG[a];
+  }
+
+  BH_PHASE();
+  // right-hand sides
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = a; b < 3; b++) {
+      const int ab = sy(a, b);
+      double lg = -g[ab] * tdb, lA = -A[ab] * tdb;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        lg += db[a][c] * g[sy(b, c)] + db[b][c] * g[sy(a, c)];
+        lA += db[a][c] * A[sy(b, c)] + db[b][c] * A[sy(a, c)];
+      }
+      rhs[ab] += -2.0 * alpha * A[ab] + lg;
+      rhs[7 + ab] += psim4 * (-cdda[ab] + alpha * R[ab]) - (1.0 / 3.0) * g[ab] * (-trcdda + alpha * Rhat) +
+                     alpha * (K_ * A[ab] - 2.0 * AA[ab]) + lA;
+    }
+  double rG[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    rG[a] = -2.0 * (Ainv[sy(a, 0)] * da[0] + Ainv[sy(a, 1)] * da[1] + Ainv[sy(a, 2)] * da[2]) +
+            2.0 * alpha * divAinv[a] - 2.0 * alpha * K.kappa1 * (Gm[a] - Gfg[a]) + pG[a];
+    rhs[14 + a] = rG[a];
+  }
+  rhs[13] += -trcdda + alpha * (cAA + K_ * K_ / 3.0) + K.kappa1 * (1.0 - K.kappa2) * alpha * Theta;
+  rhs[6] += (K.chipsipower / 6.0) * chig * divbeta - (K.chipsipower / 6.0) * chig * alpha * K_;
+  rhs[17] = (alpha * (0.5 * Hhat - (2.0 + K.kappa2) * K.kappa1 * Theta) + rhs[17]) * K.zeroThetaRHS;
+
+  BH_PHASE();
+  // gauge
+  const double falpha =
+      K.oploglapse * K.lapseharmonicf + K.oploglapse2 * (8.0 / 3.0) / (3.0 - alpha) + K.harmoniclapse * alpha;
+  rhs[18] -= falpha * alpha * Khat;
+  if (K.withB != 0.0) {
+    const double betaF = K.shiftgammacoeff * pow(alpha, K.shiftalphapower);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      rhs[19 + a] += (K.gamma2factor + K.gamma0factor * betaF) * Bv[a];
+      rhs[22 + a] += (K.gamma0factor + K.gamma2factor * betaF) * (rG[a] - K.withGadv * advG[a]) -
+                     K.shiftdriver * Bv[a];
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      rhs[22 + a] = 0.0;
+      rhs[19 + a] += Gm[a] - K.shiftdriver * beta[a];
+    }
+  }
+
+  BH_PHASE();
+  // Kreiss-Oliger dissipation
+  if (dv.ko_on(K)) {
+#pragma unroll
+    for (int v = 0; v < 25; v++) rhs[v] += K.dissfactor * dv.ko(u[v]);
+  }
+
+  bool finite = true;
+#pragma unroll
+  for (int v = 0; v < 25; v++) finite = finite && isfinite(rhs[v]);
+
+  store_point(P, K, q, rhs);
+  return finite;
+}
+
+// ------------------------------------------------------- bulk kernel (LDS)
+// Output tile of TX x TY x TZ points per block, one point per thread.  Each
+// of the 25 grid functions is staged in turn into an LDS tile with a halo of
+// 3 points (the reach of the lop-sided advection stencil), so every stencil
+// neighbour is an immediate-offset ds_read_b64 from one per-thread LDS
+// address.  The algebra is ordered by staged field: each field's
+// derivatives, advection and dissipation are consumed as soon as the field
+// is in LDS.
+constexpr int TX = 16, TY = 4, TZ = 4, HALO = 3;
+constexpr int SX = TX + 2 * HALO, SY = TY + 2 * HALO, SZ = TZ + 2 * HALO;
+constexpr int TILE = SX * SY * SZ;
+constexpr int NTHR = TX * TY * TZ;
+
+template <int DX, int DY, int DZ>
+__device__ __forceinline__ double at(const double *T) {
+  return T[(DX + HALO) + SX * ((DY + HALO) + SY * (DZ + HALO))];
+}
+// neighbour at offset o along direction A
+template <int A, int O>
+__device__ __forceinline__ double ax(const double *T) {
+  return at<(A == 0 ? O : 0), (A == 1 ? O : 0), (A == 2 ? O : 0)>(T);
+}
+template <int A>
+__device__ __forceinline__ double t_d1(const double *T, const Z4cGeo &G) {
+  return ((ax<A, -2>(T) - ax<A, 2>(T)) + 8.0 * (ax<A, 1>(T) - ax<A, -1>(T))) * G.i12h[A];
+}
+template <int A>
+__device__ __forceinline__ double t_d2(const double *T, const Z4cGeo &G) {
+  return (16.0 * (ax<A, -1>(T) + ax<A, 1>(T)) - (ax<A, -2>(T) + ax<A, 2>(T)) - 30.0 * ax<A, 0>(T)) * G.i12hh[A];
+}
+// mixed derivative d_A d_B (A < B): tensor product of 4th-order first differences
+template <int A, int B, int OA>
+__device__ __forceinline__ double t_row(const double *T) {
+  constexpr int X0 = (A == 0 ? OA : 0), Y0 = (A == 1 ? OA : 0), Z0 = (A == 2 ? OA : 0);
+  constexpr int BX = (B == 0), BY = (B == 1), BZ = (B == 2);
+  return (at<X0 - 2 * BX, Y0 - 2 * BY, Z0 - 2 * BZ>(T) - at<X0 + 2 * BX, Y0 + 2 * BY, Z0 + 2 * BZ>(T)) +
+         8.0 * (at<X0 + BX, Y0 + BY, Z0 + BZ>(T) - at<X0 - BX, Y0 - BY, Z0 - BZ>(T));
+}
+template <int A, int B>
+__device__ __forceinline__ double t_dd(const double *T, const Z4cGeo &G) {
+  return ((t_row<A, B, -2>(T) - t_row<A, B, 2>(T)) + 8.0 * (t_row<A, B, 1>(T) - t_row<A, B, -1>(T))) *
+         G.i144hh[A + B - 1];
+}
+// lop-sided advection: pw = max(beta/h,0)/12, mw = min(beta/h,0)/12 per direction
+template <int A>
+__device__ __forceinline__ double t_adv1(const double *T, double pw, double mw) {
+  return pw * ((18.0 * ax<A, 1>(T) - 3.0 * ax<A, -1>(T)) + (ax<A, 3>(T) - 6.0 * ax<A, 2>(T)) -
+               10.0 * ax<A, 0>(T)) +
+         mw * ((6.0 * ax<A, -2>(T) - ax<A, -3>(T)) + (3.0 * ax<A, 1>(T) - 18.0 * ax<A, -1>(T)) +
+               10.0 * ax<A, 0>(T));
+}
+__device__ __forceinline__ double t_adv(const double *T, const double pw[3], const double mw[3]) {
+  return t_adv1<0>(T, pw[0], mw[0]) + t_adv1<1>(T, pw[1], mw[1]) + t_adv1<2>(T, pw[2], mw[2]);
+}
+template <int A>
+__device__ __forceinline__ double t_ko1(const double *T) {
+  return (ax<A, -2>(T) + ax<A, 2>(T)) - 4.0 * (ax<A, -1>(T) + ax<A, 1>(T)) + 6.0 * ax<A, 0>(T);
+}
+__device__ __forceinline__ double t_ko(const double *T, const Z4cGeo &G) {
+  return -(t_ko1<0>(T) * G.ih[0] + t_ko1<1>(T) * G.ih[1] + t_ko1<2>(T) * G.ih[2]);
+}
+__device__ __forceinline__ void t_grad(const double *T, const Z4cGeo &G, double d[3]) {
+  d[0] = t_d1<0>(T, G);
+  d[1] = t_d1<1>(T, G);
+  d[2] = t_d1<2>(T, G);
+}
+// all six second derivatives (xx xy xz yy yz zz)
+__device__ __forceinline__ void t_hess(const double *T, const Z4cGeo &G, double d[6]) {
+  d[0] = t_d2<0>(T, G);
+  d[1] = t_dd<0, 1>(T, G);
+  d[2] = t_dd<0, 2>(T, G);
+  d[3] = t_d2<1>(T, G);
+  d[4] = t_dd<1, 2>(T, G);
+  d[5] = t_d2<2>(T, G);
+}
+
+__device__ __forceinline__ void stage(double *tile, const double *__restrict__ f, const Z4cGeo &G, int i0, int j0,
+                                      int k0, int tid) {
+  __syncthreads();
+  for (int idx = tid; idx < TILE; idx += NTHR) {
+    const int x = idx % SX, y = (idx / SX) % SY, z = idx / (SX * SY);
+    const int gi = i0 - HALO + x, gj = j0 - HALO + y, gk = k0 - HALO + z;
+    double v = 0.0;
+    if (gi < G.m && gj < G.n && gk < G.o) v = f[G.noffset + gi + G.sj * gj + G.sk * gk];
+    tile[idx] = v;
+  }
+  __syncthreads();
+}
+
+// Pin a value in registers at this point of the program: an empty volatile
+// asm that reads and writes it.  Placed after each staged field, it keeps the
+// compiler from sinking that field's arithmetic past the next stage's
+// barrier, which would keep every staged tile's values live to the end of
+// the kernel (and spill).
+__device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
+template <int N>
+__device__ __forceinline__ void pin(double (&a)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; i++) pin(a[i]);
+}
+
+__global__ void __launch_bounds__(NTHR) z4c_rhs_bulk(Z4cPtrs P, Z4cGeo G, Z4cK K, int *flag) {
+  __shared__ double tile[TILE];
+  const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
+  const int tid = tx + TX * (ty + TY * tz);
+  const int i0 = G.lo[0] + blockIdx.x * TX, j0 = G.lo[1] + blockIdx.y * TY, k0 = G.lo[2] + blockIdx.z * TZ;
+  const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
+  const bool active = (i <= G.hi[0] && j <= G.hi[1] && k <= G.hi[2]);
+  const long q = G.noffset + i + G.sj * j + G.sk * k;
+  const double *T = tile + tx + SX * (ty + SY * tz);
+  const double *const *u = P.u;
+
+  // point values; threads outside the box use flat data so that their
+  // (discarded) arithmetic stays finite
+  double g[6], A[6], Gm[3], beta[3], Bv[3];
+  double chi = 1.0, Khat = 0.0, Theta = 0.0, alpha = 1.0;
+  for (int c = 0; c < 6; c++) {
+    g[c] = (c == 0 || c == 3 || c == 5) ? 1.0 : 0.0;
+    A[c] = 0.0;
+  }
+  for (int a = 0; a < 3; a++) Gm[a] = beta[a] = Bv[a] = 0.0;
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      g[c] = u[c][q];
+      A[c] = u[7 + c][q];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      Gm[a] = u[14 + a][q];
+      beta[a] = u[19 + a][q];
+      Bv[a] = u[22 + a][q];
+    }
+    chi = u[6][q];
+    Khat = u[13][q];
+    Theta = u[17][q];
+    alpha = u[18][q];
+  }
+  double pw[3], mw[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const double w = beta[a] * G.ih[a];
+    pw[a] = (w > 0.0 ? w : 0.0) * (1.0 / 12.0);
+    mw[a] = (w < 0.0 ? w : 0.0) * (1.0 / 12.0);
+  }
+  const bool ko = (K.order_dissipation == 4);
+  const double eps = ko ? K.dissfactor : 0.0;
+
+  // inverse conformal metric
+  double ig[6];
+  ig[0] = g[3] * g[5] - g[4] * g[4];
+  ig[1] = g[2] * g[4] - g[1] * g[5];
+  ig[2] = g[1] * g[4] - g[2] * g[3];
+  ig[3] = g[0] * g[5] - g[2] * g[2];
+  ig[4] = g[1] * g[2] - g[0] * g[4];
+  ig[5] = g[0] * g[3] - g[1] * g[1];
+  const double detginv = 1.0 / (g[0] * ig[0] + g[1] * ig[1] + g[2] * ig[2]);
+#pragma unroll
+  for (int c = 0; c < 6; c++) ig[c] *= detginv;
+  auto trace = [&](const double *t6) {
+    return ig[0] * t6[0] + ig[3] * t6[3] + ig[5] * t6[5] + 2.0 * (ig[1] * t6[1] + ig[2] * t6[2] + ig[4] * t6[4]);
+  };
+  auto contract = [&](const double *x, const double *y) {
+    return ig[0] * x[0] * y[0] + ig[3] * x[1] * y[1] + ig[5] * x[2] * y[2] + ig[1] * (x[0] * y[1] + x[1] * y[0]) +
+           ig[2] * (x[0] * y[2] + x[2] * y[0]) + ig[4] * (x[1] * y[2] + x[2] * y[1]);
+  };
+
+  double rhs[25];
+  double R[6];  // Ricci accumulators (conformal + phi part)
+
+  // ---- conformal metric: first derivatives, Laplacian part of Ricci
+  double dg[3][6];
+static_for<6>([&](auto I_) {
+    constexpr int ab = decltype(I_)::value;
+    stage(tile, u[ab], G, i0, j0, k0, tid);
+    dg[0][ab] = t_d1<0>(T, G);
+    dg[1][ab] = t_d1<1>(T, G);
+    dg[2][ab] = t_d1<2>(T, G);
+    double h6[6];
+    t_hess(T, G, h6);
+    R[ab] = -0.5 * trace(h6);
+    rhs[ab] = t_adv(T, pw, mw) + eps * t_ko(T, G);
+    pin(dg[0][ab]);
+    pin(dg[1][ab]);
+    pin(dg[2][ab]);
+    pin(R[ab]);
+    pin(rhs[ab]);
+  });
+  // Christoffels
+  double Gu[3][6], Gfg[3];
+  {
+    double Gd[3][6];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = a; b < 3; b++)
+          Gd[c][sy(a, b)] = 0.5 * (dg[a][sy(b, c)] + dg[b][sy(a, c)] - dg[c][sy(a, b)]);
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int ab = 0; ab < 6; ab++)
+        Gu[c][ab] = ig[sy(c, 0)] * Gd[0][ab] + ig[sy(c, 1)] * Gd[1][ab] + ig[sy(c, 2)] * Gd[2][ab];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+      Gfg[a] = ig[0] * Gu[a][0] + ig[3] * Gu[a][3] + ig[5] * Gu[a][5] +
+               2.0 * (ig[1] * Gu[a][1] + ig[2] * Gu[a][2] + ig[4] * Gu[a][4]);
+    // quadratic Christoffel terms of R_ab
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = a; b < 3; b++) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            double t = 0.0;
+#pragma unroll
+            for (int e = 0; e < 3; e++)
+              t += Gu[e][sy(c, a)] * Gd[b][sy(e, d)] + Gu[e][sy(c, b)] * Gd[a][sy(e, d)] +
+                   Gu[e][sy(a, d)] * Gd[e][sy(c, b)];
+            s += ig[sy(c, d)] * t;
+          }
+        double t2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) t2 += Gfg[c] * (Gd[a][sy(b, c)] + Gd[b][sy(a, c)]);
+        R[sy(a, b)] += s + 0.5 * t2;
+      }
+  }
+  pin(R);
+  pin(Gu[0]);
+  pin(Gu[1]);
+  pin(Gu[2]);
+  pin(Gfg);
+
+  // ---- Gamma^i: + 1/2 (g_ca d_b G^c + g_cb d_a G^c) in R_ab
+  double advG[3], koG[3];
+static_for<3>([&](auto I_) {
+    constexpr int c = decltype(I_)::value;
+    stage(tile, u[14 + c], G, i0, j0, k0, tid);
+    double dG[3];
+    t_grad(T, G, dG);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = a; b < 3; b++) R[sy(a, b)] += 0.5 * (g[sy(c, a)] * dG[b] + g[sy(c, b)] * dG[a]);
+    advG[c] = t_adv(T, pw, mw);
+    koG[c] = eps * t_ko(T, G);
+    pin(R);
+    pin(advG[c]);
+    pin(koG[c]);
+  });
+
+  // ---- conformal factor
+  const double chig = (chi < K.chiDivFloor) ? K.chiDivFloor : chi;
+  const double oocp = 1.0 / K.chipsipower;
+  const double psim4 = exp(-4.0 * oocp * log(chig));
+  const double oochig = 1.0 / chig;
+  double dchi[3], df[3];
+  {
+    stage(tile, u[6], G, i0, j0, k0, tid);
+    t_grad(T, G, dchi);
+    double h6[6];
+    t_hess(T, G, h6);
+    rhs[6] = t_adv(T, pw, mw) + eps * t_ko(T, G);
+#pragma unroll
+    for (int a = 0; a < 3; a++) df[a] = oocp * dchi[a] * oochig;
+    double cddf[6];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = a; b < 3; b++) {
+        const int ab = sy(a, b);
+        cddf[ab] = oocp * h6[ab] * oochig - K.chipsipower * df[a] * df[b] -
+                   (Gu[0][ab] * df[0] + Gu[1][ab] * df[1] + Gu[2][ab] * df[2]);
+      }
+    const double trcddf = trace(cddf);
+    const double dfdf = contract(df, df);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = a; b < 3; b++) {
+        const int ab = sy(a, b);
+        R[ab] += 4.0 * df[a] * df[b] - 2.0 * cddf[ab] - 2.0 * g[ab] * trcddf - 4.0 * g[ab] * dfdf;
+      }
+    pin(R);
+    pin(dchi);
+    pin(df);
+    pin(rhs[6]);
+  }
+
+  // ---- lapse
+  double da[3], cdda[6];
+  {
+    stage(tile, u[18], G, i0, j0, k0, tid);
+    t_grad(T, G, da);
+    double h6[6];
+    t_hess(T, G, h6);
+    rhs[18] = K.oplogwithshift * t_adv(T, pw, mw) + eps * t_ko(T, G);
+    const double dfda = contract(df, da);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = a; b < 3; b++) {
+        const int ab = sy(a, b);
+        cdda[ab] = h6[ab] - (Gu[0][ab] * da[0] + Gu[1][ab] * da[1] + Gu[2][ab] * da[2]) -
+                   2.0 * (df[a] * da[b] + df[b] * da[a] - g[ab] * dfda);
+      }
+    pin(da);
+    pin(cdda);
+    pin(rhs[18]);
+  }
+  const double trcdda = psim4 * trace(cdda);
+
+  // ---- Khat and Theta: 2 dKhat + dTheta
+  double dKT[3], koT;
+  {
+    stage(tile, u[13], G, i0, j0, k0, tid);
+    dKT[0] = 2.0 * t_d1<0>(T, G);
+    dKT[1] = 2.0 * t_d1<1>(T, G);
+    dKT[2] = 2.0 * t_d1<2>(T, G);
+    rhs[13] = t_adv(T, pw, mw) + eps * t_ko(T, G);
+    pin(dKT);
+    pin(rhs[13]);
+    stage(tile, u[17], G, i0, j0, k0, tid);
+    dKT[0] += t_d1<0>(T, G);
+    dKT[1] += t_d1<1>(T, G);
+    dKT[2] += t_d1<2>(T, G);
+    rhs[17] = t_adv(T, pw, mw);  // KO of Theta is added after the zeroThetaRHS factor
+    koT = eps * t_ko(T, G);
+    pin(dKT);
+    pin(rhs[17]);
+    pin(koT);
+  }
+
+  // ---- trace-free extrinsic curvature: advection and dissipation only
+static_for<6>([&](auto I_) {
+    constexpr int ab = decltype(I_)::value;
+    stage(tile, u[7 + ab], G, i0, j0, k0, tid);
+    rhs[7 + ab] = t_adv(T, pw, mw) + eps * t_ko(T, G);
+    pin(rhs[7 + ab]);
+  });
+
+  // ---- shift: d_a beta^b, g^bc d_b d_c beta^a, (1/3) d_a d_c beta^c
+  double db[3][3], pG[3], ootd[3];
+static_for<3>([&](auto I_) {
+    constexpr int c = decltype(I_)::value;
+    stage(tile, u[19 + c], G, i0, j0, k0, tid);
+    db[0][c] = t_d1<0>(T, G);
+    db[1][c] = t_d1<1>(T, G);
+    db[2][c] = t_d1<2>(T, G);
+    double h6[6];
+    t_hess(T, G, h6);
+    pG[c] = trace(h6);
+#pragma unroll
+    for (int a = 0; a < 3; a++) ootd[a] = (c == 0 ? 0.0 : ootd[a]) + (1.0 / 3.0) * h6[sy(a, c)];
+    rhs[19 + c] = K.withShiftadv * t_adv(T, pw, mw) + eps * t_ko(T, G);
+    pin(db[0][c]);
+    pin(db[1][c]);
+    pin(db[2][c]);
+    pin(pG[c]);
+    pin(ootd);
+    pin(rhs[19 + c]);
+  });
+  // ---- B: advection and dissipation only
+static_for<3>([&](auto I_) {
+    constexpr int c = decltype(I_)::value;
+    stage(tile, u[22 + c], G, i0, j0, k0, tid);
+    rhs[22 + c] = (K.withB != 0.0 ? K.withBadv * t_adv(T, pw, mw) : 0.0) + eps * t_ko(T, G);
+    pin(rhs[22 + c]);
+  });
+
+  // ---- algebra
+  double Ainv[6], AA[6];
+  {
+    double Au[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++)
+        Au[a][b] = ig[sy(a, 0)] * A[sy(0, b)] + ig[sy(a, 1)] * A[sy(1, b)] + ig[sy(a, 2)] * A[sy(2, b)];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = a; b < 3; b++) {
+        Ainv[sy(a, b)] = Au[a][0] * ig[sy(0, b)] + Au[a][1] * ig[sy(1, b)] + Au[a][2] * ig[sy(2, b)];
+        AA[sy(a, b)] = A[sy(a, 0)] * Au[0][b] + A[sy(a, 1)] * Au[1][b] + A[sy(a, 2)] * Au[2][b];
+      }
+  }
+  const double cAA = trace(AA);
+  const double K_ = Khat + 2.0 * Theta;
+  double divAinv[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    double s = 0.0;
+#pragma unroll
+    for (int bc = 0; bc < 6; bc++) s += ((bc == 0 || bc == 3 || bc == 5) ? 1.0 : 2.0) * Gu[a][bc] * Ainv[bc];
+    s -= 1.5 * oochig * (Ainv[sy(a, 0)] * dchi[0] + Ainv[sy(a, 1)] * dchi[1] + Ainv[sy(a, 2)] * dchi[2]);
+    s -= (1.0 / 3.0) * (ig[sy(a, 0)] * dKT[0] + ig[sy(a, 1)] * dKT[1] + ig[sy(a, 2)] * dKT[2]);
+    divAinv[a] = s;
+  }
+  const double Rhat = psim4 * trace(R);
+  const double Hhat = Rhat + 2.0 * K_ * K_ / 3.0 - cAA;
+  const double divbeta = db[0][0] + db[1][1] + db[2][2];
+  const double tdb = (2.0 / 3.0) * divbeta;
+
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = a; b < 3; b++) {
+      const int ab = sy(a, b);
+      double lg = -g[ab] * tdb, lA = -A[ab] * tdb;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        lg += db[a][c] * g[sy(b, c)] + db[b][c] * g[sy(a, c)];
+        lA += db[a][c] * A[sy(b, c)] + db[b][c] * A[sy(a, c)];
+      }
+      rhs[ab] += -2.0 * alpha * A[ab] + lg;
+      rhs[7 + ab] += psim4 * (-cdda[ab] + alpha * R[ab]) - (1.0 / 3.0) * g[ab] * (-trcdda + alpha * Rhat) +
+                     alpha * (K_ * A[ab] - 2.0 * AA[ab]) + lA;
+    }
+  double rG[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const double pseudolie = pG[a] + ig[sy(a, 0)] * ootd[0] + ig[sy(a, 1)] * ootd[1] + ig[sy(a, 2)] * ootd[2] -
+                             (Gfg[0] * db[0][a] + Gfg[1] * db[1][a] + Gfg[2] * db[2][a]) + Gfg[a] * tdb + advG[a];
+    rG[a] = -2.0 * (Ainv[sy(a, 0)] * da[0] + Ainv[sy(a, 1)] * da[1] + Ainv[sy(a, 2)] * da[2]) +
+            2.0 * alpha * divAinv[a] - 2.0 * alpha * K.kappa1 * (Gm[a] - Gfg[a]) + pseudolie;
+    rhs[14 + a] = rG[a] + koG[a];
+  }
+  rhs[13] += -trcdda + alpha * (cAA + K_ * K_ / 3.0) + K.kappa1 * (1.0 - K.kappa2) * alpha * Theta;
+  rhs[6] += (K.chipsipower / 6.0) * chig * (divbeta - alpha * K_);
+  rhs[17] = (alpha * (0.5 * Hhat - (2.0 + K.kappa2) * K.kappa1 * Theta) + rhs[17]) * K.zeroThetaRHS + koT;
+
+  const double falpha =
+      K.oploglapse * K.lapseharmonicf + K.oploglapse2 * (8.0 / 3.0) / (3.0 - alpha) + K.harmoniclapse * alpha;
+  rhs[18] -= falpha * alpha * Khat;
+  if (K.withB != 0.0) {
+    const double betaF = K.shiftgammacoeff * pow(alpha, K.shiftalphapower);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      rhs[19 + a] += (K.gamma2factor + K.gamma0factor * betaF) * Bv[a];
+      rhs[22 + a] += (K.gamma0factor + K.gamma2factor * betaF) * (rG[a] - K.withGadv * advG[a]) -
+                     K.shiftdriver * Bv[a];
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; a++) rhs[19 + a] += Gm[a] - K.shiftdriver * beta[a];
+  }
+  if (!active) return;
+  bool finite = true;
+#pragma unroll
+  for (int v = 0; v < 25; v++) finite = finite && isfinite(rhs[v]);
+  if (!finite) *flag = 1;
+  store_point(P, K, q, rhs);
+}
+
+// ------------------------------------------------------------------ kernels
+// all computed points that are not in [blo, bhi] (the bulk region), or all
+// computed points when edge_only == 0
+__global__ void __launch_bounds__(256) z4c_rhs_edge(Z4cPtrs P, Z4cGeo G, Z4cK K, int blo0, int blo1, int blo2,
+                                                   int bhi0, int bhi1, int bhi2, int *flag) {
+  const int i = G.lo[0] + blockIdx.x * 64 + threadIdx.x;
+  const int j = G.lo[1] + blockIdx.y * 4 + threadIdx.y;
+  const int k = G.lo[2] + blockIdx.z;
+  if (i > G.hi[0] || j > G.hi[1] || k > G.hi[2]) return;
+  if (G.edge_only && i >= blo0 && i <= bhi0 && j >= blo1 && j <= bhi1 && k >= blo2 && k <= bhi2) return;
+  const long q = G.noffset + i + G.sj * j + G.sk * k;
+  Edge dv(G, q, i, j, k, K);
+  if (!z4c_point(dv, P, K, q)) *flag = 1;
+}
+
+__global__ void set_int(int *p, int v) { *p = v; }
+
+static Z4cK make_k(const bamhip_z4c_params *p, double c) {
+  Z4cK K;
+  K.order_centered = p->order_centered;
+  K.order_advection = p->order_advection;
+  K.lopsided = p->advection_lopsided;
+  K.order_dissipation = p->order_dissipation;
+  K.dissfactor = p->dissfactor;
+  K.kappa1 = p->kappa1;
+  K.kappa2 = p->kappa2;
+  K.zeroThetaRHS = p->zeroThetaRHS;
+  K.subtractA = p->subtractA;
+  K.normalizedetg = p->normalizedetg;
+  K.oploglapse = p->oploglapse;
+  K.oploglapse2 = p->oploglapse2;
+  K.oplogwithshift = p->oplogwithshift;
+  K.harmoniclapse = p->harmoniclapse;
+  K.lapseharmonicf = p->lapseharmonicf;
+  K.gamma0factor = p->gamma0factor;
+  K.gamma2factor = p->gamma2factor;
+  K.withGadv = p->withGadv;
+  K.withShiftadv = p->withShiftadv;
+  K.withBadv = p->withBadv;
+  K.withB = p->withB;
+  K.shiftalphapower = p->shiftalphapower;
+  K.shiftgammacoeff = p->shiftgammacoeff;
+  K.shiftdriver = p->shiftdriver;
+  K.chiDivFloor = p->chiDivFloor;
+  K.chipsipower = p->chipsipower;
+  K.c = c;
+  K.addlinear = (c != 0.0);
+  return K;
+}
+
+static void make_geo(const DevBox &B, Z4cGeo &G) {
+  G.m = B.m;
+  G.n = B.n;
+  G.o = B.o;
+  G.noffset = B.noffset;
+  G.lo[0] = B.skip[0];
+  G.hi[0] = B.m - 1 - B.skip[1];
+  G.lo[1] = B.skip[2];
+  G.hi[1] = B.n - 1 - B.skip[3];
+  G.lo[2] = B.skip[4];
+  G.hi[2] = B.o - 1 - B.skip[5];
+  G.sj = B.m;
+  G.sk = (long)B.m * B.n;
+  const double h[3] = {B.dx, B.dy, B.dz};
+  for (int a = 0; a < 3; a++) {
+    G.h[a] = h[a];
+    G.ih[a] = 1.0 / h[a];
+    G.i12h[a] = 1.0 / (12.0 * h[a]);
+    G.i12hh[a] = 1.0 / (12.0 * h[a] * h[a]);
+    G.i2h[a] = 1.0 / (2.0 * h[a]);
+    G.ihh[a] = 1.0 / (h[a] * h[a]);
+  }
+  G.i144hh[0] = 1.0 / (144.0 * h[0] * h[1]);
+  G.i144hh[1] = 1.0 / (144.0 * h[0] * h[2]);
+  G.i144hh[2] = 1.0 / (144.0 * h[1] * h[2]);
+  G.i4hh[0] = 1.0 / (4.0 * h[0] * h[1]);
+  G.i4hh[1] = 1.0 / (4.0 * h[0] * h[2]);
+  G.i4hh[2] = 1.0 / (4.0 * h[1] * h[2]);
+  G.edge_only = 0;
+}
+
+// launch the RHS over one box; returns error code
+static int launch_box(const DevBox &B, const Z4cPtrs &P, const Z4cK &K, int *flag) {
+  Z4cGeo G;
+  make_geo(B, G);
+  if (G.hi[0] < G.lo[0] || G.hi[1] < G.lo[1] || G.hi[2] < G.lo[2]) return BAMHIP_OK;
+  const bool fast = (K.order_centered == 4 && K.order_advection == 4 && K.lopsided == 1 &&
+                     (K.order_dissipation == 4 || K.order_dissipation == 0));
+  // bulk region: >= 3 points from every box edge
+  int blo[3], bhi[3];
+  const int dims[3] = {B.m, B.n, B.o};
+  bool have_bulk = fast;
+  for (int a = 0; a < 3; a++) {
+    blo[a] = G.lo[a] > 3 ? G.lo[a] : 3;
+    bhi[a] = G.hi[a] < dims[a] - 4 ? G.hi[a] : dims[a] - 4;
+    if (bhi[a] < blo[a]) have_bulk = false;
+  }
+  if (have_bulk) {
+    Z4cGeo Gb = G;
+    for (int a = 0; a < 3; a++) {
+      Gb.lo[a] = blo[a];
+      Gb.hi[a] = bhi[a];
+    }
+    dim3 grid((bhi[0] - blo[0] + TX) / TX, (bhi[1] - blo[1] + TY) / TY, (bhi[2] - blo[2] + TZ) / TZ);
+    {
+      TimerScope t("z4_rhs");
+      z4c_rhs_bulk<<<grid, dim3(TX, TY, TZ), 0, stream()>>>(P, Gb, K, flag);
+    }
+    BH_HIP(hipGetLastError());
+  }
+  const bool need_edge = !have_bulk || blo[0] > G.lo[0] || bhi[0] < G.hi[0] || blo[1] > G.lo[1] ||
+                         bhi[1] < G.hi[1] || blo[2] > G.lo[2] || bhi[2] < G.hi[2];
+  if (need_edge) {
+    G.edge_only = have_bulk ? 1 : 0;
+    dim3 grid((G.hi[0] - G.lo[0] + 64) / 64, (G.hi[1] - G.lo[1] + 4) / 4, G.hi[2] - G.lo[2] + 1);
+    {
+      TimerScope t("z4_rhs_edge");
+      z4c_rhs_edge<<<grid, dim3(64, 4, 1), 0, stream()>>>(P, G, K, blo[0], blo[1], blo[2], bhi[0], bhi[1], bhi[2],
+                                                          flag);
+    }
+    BH_HIP(hipGetLastError());
+  }
+  return BAMHIP_OK;
+}
+
+int z4c_rhs_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, const int *upre, double c,
+                 const int *ucur, bool check_finite) {
+  if (!lev || !par) return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: NULL argument");
+  if (par->order_centered != 2 && par->order_centered != 4)
+    return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: order_centered=%d not implemented (2, 4)", par->order_centered);
+  if (par->order_advection != 2 && par->order_advection != 4)
+    return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: order_advection=%d not implemented (2, 4)",
+                     par->order_advection);
+  if (par->advection_lopsided < 0 || par->advection_lopsided > 2)
+    return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: advection_lopsided=%d", par->advection_lopsided);
+  if (par->order_dissipation != 0 && par->order_dissipation != 4)
+    return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: order_dissipation=%d not implemented (0, 4)",
+                     par->order_dissipation);
+  Z4cPtrs P;
+  int e;
+  if ((e = gather_ptrs(lev, ucur, 25, const_cast<double **>(P.u), "bamhip_z4c_rhs(ucur)"))) return e;
+  if ((e = gather_ptrs(lev, unew, 25, P.r, "bamhip_z4c_rhs(unew)"))) return e;
+  if (c != 0.0) {
+    if ((e = gather_ptrs(lev, upre, 25, const_cast<double **>(P.p), "bamhip_z4c_rhs(upre)"))) return e;
+  } else {
+    for (int v = 0; v < 25; v++) P.p[v] = P.u[v];
+  }
+  for (int v = 0; v < 25; v++)
+    for (int w = 0; w < 25; w++)
+      if (P.r[v] == P.u[w])
+        return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: unew[%d] aliases ucur[%d] (stencils need the old data)", v, w);
+  // the stencil reach (3 with lop-sided advection, 4 with one-sided) must stay
+  // inside every box
+  for (const DevBox &B : lev->boxes)
+    for (int f = 0; f < 6; f++)
+      if (B.skip[f] < 1) return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: box face %d has no flagged layer", f);
+  const Z4cK K = make_k(par, c);
+  if (check_finite) set_int<<<1, 1, 0, stream()>>>(lev->d_flag, 0);
+  for (const DevBox &B : lev->boxes)
+    if ((e = launch_box(B, P, K, lev->d_flag))) return e;
+  if (check_finite) {
+    int h = 0;
+    BH_HIP(hipMemcpyAsync(&h, lev->d_flag, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    BH_HIP(hipStreamSynchronize(stream()));
+    if (h) return set_error(BAMHIP_ENONFINITE, "bamhip_z4c_rhs: non-finite right-hand side on level %d", lev->l);
+  }
+  return BAMHIP_OK;
+}
+
+}  // namespace bamhip
+
+extern "C" int bamhip_z4c_rhs(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, const int *upre,
+                              double c, const int *ucur) {
+  return bamhip::z4c_rhs_impl(lev, par, unew, upre, c, ucur, lev ? lev->check_mode != 0 : true);
+}

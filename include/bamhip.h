@@ -1,0 +1,178 @@
+/* bamhip.h -- C ABI of the MI355X (gfx950) Z4c / RK4 / FMR / con2prim path.
+ *
+ * The library keeps a device mirror of BAM levels.  A level is a set of boxes
+ * whose points are concatenated in one linear index space (tL::box[b]->noffset,
+ * src/main/amr/bam_amr.h:10-75) and, per BAM variable index, one device array
+ * of doubles of length npoints (the device counterpart of tL::v[index],
+ * enabled on demand like enablevarcomp, src/main/main/storage.c:79-95).
+ * Variable lists are passed as plain int arrays of BAM variable indices: the
+ * `index` member of tVarList (bam_amr.h:127-131).
+ *
+ * All functions return 0 on success and a negative code on error; the text
+ * of the last error is available from bamhip_last_error().  Errors that BAM
+ * reports with errorexit() (non-finite RHS, c2p failure with no atmosphere,
+ * boxes outside their parent) are returned as codes instead.
+ *
+ * Threading: one host thread per device, like BAM's one MPI rank per process.
+ * All device work is issued on the library's stream, which the caller can
+ * query to order its own work.  Entry points synchronise only where they
+ * return data to the host.
+ */
+#ifndef BAMHIP_H
+#define BAMHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  BAMHIP_OK = 0,
+  BAMHIP_EINVAL = -1,    /* bad argument (size, index, parameter) */
+  BAMHIP_EDEVICE = -2,   /* HIP runtime error */
+  BAMHIP_ENOMEM = -3,
+  BAMHIP_ENONFINITE = -4 /* non-finite values produced (errorexit in BAM) */
+};
+
+/* ---------------------------------------------------------------- runtime */
+/* select the device and create the library stream (bampi_initialize) */
+int bamhip_init(int device);
+int bamhip_finalize(void);
+const char *bamhip_last_error(void);
+/* the hipStream_t all library work runs on */
+void *bamhip_stream(void);
+int bamhip_synchronize(void);
+/* device buffers for callers without their own allocator */
+int bamhip_malloc(void **ptr, size_t bytes);
+int bamhip_free(void *ptr);
+int bamhip_memcpy_htod(void *dst, const void *src, size_t bytes);
+int bamhip_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+
+/* ------------------------------------------------------------------ levels */
+/* One box (tB, src/main/amr/bam_amr.h:10-31).  skip[f] = number of flagged
+ * point layers at face f (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z): the flags BAM keeps
+ * in tL::boundary (PHYBOUND = 1 layer, PERBOUND = 2, GHOBOUND = nghosts,
+ * REFBOUND = 1).  bflag[f] is the reference's per-face flag (bam_amr.h:114-122)
+ * and selects the boundary treatment in bamhip_z4c_set_boundary. */
+typedef struct {
+  int m, n, o;            /* points per direction */
+  double x0, y0, z0;      /* coordinates of point (0,0,0) */
+  double dx, dy, dz;
+  int skip[6];
+  int bflag[6];
+} bamhip_box;
+
+enum { BAMHIP_REFBOUND = -1, BAMHIP_NOTBOUND = 0, BAMHIP_PHYBOUND = 1, BAMHIP_GHOBOUND = 2,
+       BAMHIP_SYMBOUND = 3, BAMHIP_PERBOUND = 6 };
+
+typedef struct bamhip_level bamhip_level;
+
+/* create a level of nboxes boxes with room for nvariables variable indices;
+ * box b occupies points [noffset_b, noffset_b + m*n*o) with noffset
+ * accumulated in box order, as BAM does */
+int bamhip_level_create(bamhip_level **lev, int l, int nboxes, const bamhip_box *boxes,
+                        int nvariables);
+int bamhip_level_free(bamhip_level *lev);
+int bamhip_level_npoints(const bamhip_level *lev, long *npoints);
+/* enablevarcomp / disablevarcomp on the device mirror */
+int bamhip_level_enable(bamhip_level *lev, int var);
+int bamhip_level_disable(bamhip_level *lev, int var);
+/* device pointer of variable var (NULL if not enabled) */
+double *bamhip_level_ptr(bamhip_level *lev, int var);
+/* bamhip_{sync_to_device,sync_to_host}: copy npoints doubles */
+int bamhip_level_put(bamhip_level *lev, int var, const double *host);
+int bamhip_level_get(bamhip_level *lev, int var, double *host);
+int bamhip_level_fill(bamhip_level *lev, int var, double value);
+
+/* ------------------------------------------------------------------- Z4c */
+/* Snapshot of the parameters z4_rhs_movpunc_N reads on every call
+ * (src/projects/z4/z4_rhs_movpunc_N.c:108-142); flags hold the value of the
+ * corresponding Getv() test, dissfactor the value of get_dissipation_factor()
+ * (src/utility/evolve/evolve.c:489-530) for the level. */
+typedef struct {
+  int order_centered;     /* 2 or 4 */
+  int order_advection;    /* 2 or 4 */
+  int advection_lopsided; /* 0, 1 or 2 */
+  int order_dissipation;  /* 0 or 4 */
+  double dissfactor;
+  double kappa1, kappa2;
+  double zeroThetaRHS;
+  int subtractA, normalizedetg;
+  int oploglapse, oploglapse2, oplogwithshift, harmoniclapse;
+  double lapseharmonicf;
+  double gamma0factor, gamma2factor, withGadv, withShiftadv, withBadv, withB;
+  double shiftalphapower, shiftgammacoeff, shiftdriver;
+  double chiDivFloor, chipsipower;
+} bamhip_z4c_params;
+
+#define BAMHIP_Z4C_NVARS 25
+
+/* Replaces z4_rhs_movpunc_N(unew, upre, c, ucur) (z4_rhs_movpunc_N.c:23)
+ * registered through set_rhs_func_register("geometry", ...) (evolve.c:185,
+ * z4.c:245).  unew/upre/ucur: the 25 variable indices of each list in the
+ * order of z4.c:234-242.  c == 0: unew = RHS at all inner points; c != 0:
+ * unew = upre + c*RHS with trace removal / det normalisation.  upre may be
+ * NULL when c == 0.  Returns BAMHIP_ENONFINITE if any RHS value is not finite
+ * (the reference calls errorexit, z4_rhs_movpunc_N.c:10473-10479). */
+int bamhip_z4c_rhs(bamhip_level *lev, const bamhip_z4c_params *par,
+                   const int *unew, const int *upre, double c, const int *ucur);
+
+/* Replaces set_boundary(unew, upre, c, ucur) for the Z4c list
+ * (src/utility/boundary/boundary.c:25-224): radiative condition with the far
+ * limits and speeds of z4_startup (z4.c:200-228) on PHYBOUND faces, then the
+ * periodic copy of pgh layers on PERBOUND face pairs. */
+int bamhip_z4c_set_boundary(bamhip_level *lev, const bamhip_z4c_params *par,
+                            const int *unew, const int *upre, double c, const int *ucur,
+                            int pgh);
+
+/* vladd / vladdto / vlcopy over the n variables of the lists
+ * (src/main/main/variables.c:575-745): r = ca*a + cb*b on all points */
+int bamhip_vladd(bamhip_level *lev, int n, const int *r, double ca, const int *a,
+                 double cb, const int *b);
+
+/* fused vladdto + vladd of one RK4 stage (rk.c:231-252) over the n variables
+ * of the lists: uc += wc*r, then v = up + wv*r (skipped when wv == 0) */
+int bamhip_rk4_stage_update(bamhip_level *lev, int n, const int *uc, const int *up, const int *r,
+                            const int *v, double wc, double wv);
+
+/* non-finite RHS detection mode of a level: 1 (default) = bamhip_z4c_rhs
+ * synchronises and returns BAMHIP_ENONFINITE, as the reference's errorexit;
+ * 0 = no synchronisation, the flag accumulates until bamhip_level_nonfinite
+ * reads (and optionally resets) it */
+int bamhip_level_set_check(bamhip_level *lev, int mode);
+int bamhip_level_nonfinite(bamhip_level *lev, int reset, int *flag);
+
+/* Replaces evolve_rk_rk4 (src/utility/evolve/rk.c:220-257) for the Z4c list:
+ * one full RK4 step of size dt on the level with the RHS and boundary above,
+ * using fused stage kernels.  uc: state (updated), up: saved state u_p,
+ * r, v: stage registers.  Algebraic constraints (z4_algcon_wrapper) are
+ * applied afterwards when algcon != 0.  pgh: periodic layers (0 = none). */
+int bamhip_z4c_rk4_step(bamhip_level *lev, const bamhip_z4c_params *par, double dt,
+                        const int *uc, const int *up, const int *r, const int *v,
+                        int pgh, int algcon);
+
+/* z4_algcon (z4_algcon.m) on all points: g: 6 indices of gt, A: 6 of At */
+int bamhip_z4c_algcon(bamhip_level *lev, const int *g, const int *A, int subtractA,
+                      int normalizedetg);
+
+/* L2 norm sqrt(sum H^2/n) of the Hamiltonian constraint over inner points
+ * (adm_constraints_N order 4 on z4_to_adm data, reduce.c:25-30,209-217);
+ * ham: optional variable index receiving H (-1: not stored) */
+int bamhip_z4c_ham_l2(bamhip_level *lev, const int *u, double chipsipower, int ham,
+                      double *l2);
+
+/* ----------------------------------------------------------------- timers */
+/* accumulated device time per kernel family (BAM's timer_start/timer_stop
+ * keys, src/main/bampi/timer.c:60-104), measured with HIP events on the
+ * library stream; names: "z4_rhs", "z4_rhs_edge", "rk4_update", "set_boundary",
+ * "algcon", "prolong", "restrict", "c2p", "mlp" */
+int bamhip_timer_enable(int on);
+int bamhip_timer_reset(void);
+int bamhip_timer_get(const char *name, double *ms_total, long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
