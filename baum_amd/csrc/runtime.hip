@@ -46,9 +46,47 @@ static bool g_timer_on = false;
 static std::map<std::string, TimerRec> g_timers;
 static std::mutex g_timer_mu;
 
+// Events are recorded around each launch on the library stream and kept in
+// a pending list; they are resolved (synchronised) only when the timers are
+// read, so timing does not serialise the stream.  Events are pooled.
+struct Pending {
+  std::string name;
+  hipEvent_t a, b;
+};
+static std::vector<Pending> g_pending;
+static std::vector<hipEvent_t> g_pool;
+
+static hipEvent_t pool_get() {
+  if (!g_pool.empty()) {
+    hipEvent_t e = g_pool.back();
+    g_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+static void resolve_pending() {
+  for (Pending &p : g_pending) {
+    float ms = 0;
+    if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      TimerRec &r = g_timers[p.name];
+      r.ms += ms;
+      r.launches += 1;
+    }
+    g_pool.push_back(p.a);
+    g_pool.push_back(p.b);
+  }
+  g_pending.clear();
+}
+
 TimerScope::TimerScope(const char *name) : name_(name), a_(nullptr), b_(nullptr), on_(g_timer_on) {
   if (!on_) return;
-  if (hipEventCreate(&a_) != hipSuccess || hipEventCreate(&b_) != hipSuccess) {
+  std::lock_guard<std::mutex> lk(g_timer_mu);
+  a_ = pool_get();
+  b_ = pool_get();
+  if (!a_ || !b_) {
     on_ = false;
     return;
   }
@@ -58,17 +96,9 @@ TimerScope::TimerScope(const char *name) : name_(name), a_(nullptr), b_(nullptr)
 TimerScope::~TimerScope() {
   if (!on_) return;
   (void)hipEventRecord(b_, g_stream);
-  (void)hipEventSynchronize(b_);
-  float ms = 0;
-  (void)hipEventElapsedTime(&ms, a_, b_);
-  {
-    std::lock_guard<std::mutex> lk(g_timer_mu);
-    TimerRec &r = g_timers[name_];
-    r.ms += ms;
-    r.launches += 1;
-  }
-  (void)hipEventDestroy(a_);
-  (void)hipEventDestroy(b_);
+  std::lock_guard<std::mutex> lk(g_timer_mu);
+  g_pending.push_back(Pending{name_, a_, b_});
+  if (g_pending.size() > 4096) resolve_pending();
 }
 
 }  // namespace bamhip
@@ -278,6 +308,7 @@ int bamhip_timer_enable(int on) {
 
 int bamhip_timer_reset(void) {
   std::lock_guard<std::mutex> lk(g_timer_mu);
+  resolve_pending();
   g_timers.clear();
   return BAMHIP_OK;
 }
@@ -285,6 +316,7 @@ int bamhip_timer_reset(void) {
 int bamhip_timer_get(const char *name, double *ms_total, long *launches) {
   if (!name) return set_error(BAMHIP_EINVAL, "bamhip_timer_get: NULL name");
   std::lock_guard<std::mutex> lk(g_timer_mu);
+  resolve_pending();
   auto it = g_timers.find(name);
   if (ms_total) *ms_total = (it == g_timers.end()) ? 0.0 : it->second.ms;
   if (launches) *launches = (it == g_timers.end()) ? 0 : it->second.launches;
