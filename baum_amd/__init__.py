@@ -284,6 +284,86 @@ class Level:
         return out.value
 
 
+# ---------------------------------------------------------------- con2prim
+C2P_COLD, C2P_ATM, C2P_OK, C2P_FAIL, C2P_NR = 0, 1, 2, 3, 4
+C2P_HYBRID, C2P_PLAIN, C2P_BRACKETED, C2P_LEARNED = 0, 1, 2, 3
+
+
+class C2PParams(C.Structure):
+    _fields_ = [("eos", C.c_int), ("gamma", C.c_double), ("K", C.c_double),
+                ("rhoatm", C.c_double), ("fatm", C.c_double), ("epslatm", C.c_double),
+                ("patm", C.c_double), ("vmax", C.c_double), ("errormax", C.c_double),
+                ("countmax", C.c_int)]
+
+
+def c2p_params(eos="ideal", gamma=2.0, K=100.0, rhoatm=1e-20, fatm=100.0, vmax=0.999, errormax=1e-10,
+               countmax=100):
+    """EOS/atmosphere snapshot; the atmosphere state follows the cold EOS
+    (grhd_atm.c:74-75)"""
+    P = C2PParams()
+    P.eos = 0 if eos == "ideal" else 1
+    P.gamma, P.K = gamma, K
+    P.rhoatm, P.fatm = rhoatm, fatm
+    kr = K * rhoatm ** (gamma - 1.0)
+    P.patm = rhoatm * kr
+    P.epslatm = kr / (gamma - 1.0)
+    P.vmax, P.errormax, P.countmax = vmax, errormax, countmax
+    return P
+
+
+class DeviceArray:
+    """a device buffer owned by the library allocator"""
+
+    def __init__(self, shape, dtype=np.float64):
+        init()
+        self.shape = tuple(shape) if np.iterable(shape) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().bamhip_malloc(C.byref(p), self.nbytes), "bamhip_malloc")
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype or a.dtype)
+        d = cls(a.shape, a.dtype)
+        d.put(a)
+        return d
+
+    def put(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(lib().bamhip_memcpy_htod(self.ptr, a.ctypes.data, self.nbytes), "bamhip_memcpy_htod")
+
+    def get(self):
+        a = np.empty(self.shape, self.dtype)
+        check(lib().bamhip_memcpy_dtoh(a.ctypes.data, self.ptr, self.nbytes), "bamhip_memcpy_dtoh")
+        return a
+
+    def free(self):
+        if self.ptr:
+            lib().bamhip_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def c2p_batch(P, method, cons, metric, prim, status, iters):
+    """device-array c2p: cons (5,n) in/out, metric (6,n), prim (8,n) in/out
+    (prim[5] = pressure guess, prim[0] = density guess), status/iters (n,)"""
+    n = cons.shape[1]
+    L = lib()
+    L.bamhip_c2p_batch.restype = C.c_int
+    L.bamhip_c2p_batch.argtypes = [C.POINTER(C2PParams), C.c_int, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]
+    check(L.bamhip_c2p_batch(C.byref(P), method, n, cons.ptr, metric.ptr, prim.ptr, status.ptr, iters.ptr),
+          "bamhip_c2p_batch")
+
+
 # ------------------------------------------------------------------ timers
 def timer_enable(on=True):
     check(lib().bamhip_timer_enable(1 if on else 0), "bamhip_timer_enable")

@@ -1,0 +1,506 @@
+// c2p.hip -- GRHD conservative-to-primitive recovery on gfx950, FP64.
+//
+// Follows src/projects/grhd/grhd_c2p_v2.c:24-253 (try_hot_part: Newton-
+// Raphson on the pressure with the general EOS; try_cold_part: Newton-
+// Raphson on the density with the cold polytrope), :265-464 (the hybrid
+// point loop with the cold static atmosphere, which rewrites the conserved
+// variables on COLD/ATM outcomes) and src/projects/grhd/grhd_c2p.c:102-237
+// (the plain p-root without atmosphere).  EOS: ideal gas / polytrope of
+// src/projects/eos/eos_analytic.c:52-72,423-434.
+//
+// One thread per point, SoA arrays, no shared state: the kernel is bound by
+// the 21 doubles per point it moves (HBM) and by the FP64 div/sqrt of each
+// Newton iteration; iteration counts differ from point to point, so lanes of
+// a wave finish at different times (the iteration histogram is returned).
+// FP contraction is off so that every rounding matches the CPU oracle and
+// status codes and iteration counts agree exactly.
+#include <cmath>
+
+#include "bamhip_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace bamhip {
+
+enum { C2P_COLD = 0, C2P_ATM, C2P_OK, C2P_FAIL, C2P_NR };
+
+struct C2PK {
+  int eos;  // 0 ideal gas (cold part: polytrope), 1 polytrope
+  double gamma, K, rhoatm, fatm, epslatm, patm, vmax, errormax;
+  int countmax;
+};
+
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+
+// EOS.comp("re",...,"pc","re"): p(rho, eps) with chi = dp/drho, kappa = dp/deps
+__device__ __forceinline__ int eos_re(const C2PK &P, double rho, double eps, double *p, double *chi, double *kappa) {
+  if (P.eos == 0) {
+    *p = (P.gamma - 1) * rho * eps;
+    *kappa = (P.gamma - 1) * rho;
+    *chi = (P.gamma - 1) * eps;
+  } else {
+    const double kr = P.K * pow(rho, P.gamma - 1.);
+    *p = rho * kr;
+    *kappa = 0.;
+    *chi = P.gamma * kr;
+  }
+  const double cs2 = (rho == 0.) ? 0. : (*chi + *p * *kappa / (rho * rho)) / (1.0 + eps + *p / rho);
+  return (!isfinite(rho) || !isfinite(eps) || !isfinite(cs2)) ? 1 : 0;
+}
+
+// cold polytrope EOS.comp("r",...,"pe","r")
+__device__ __forceinline__ void eos_cold(const C2PK &P, double rho, double *p, double *eps, double *chi) {
+  const double kr = P.K * pow(rho, P.gamma - 1.);
+  *p = rho * kr;
+  *chi = P.gamma * kr;
+  *eps = kr / (P.gamma - 1.);
+}
+
+__device__ int try_hot(const C2PK &P, double conD, double conS2, double conT, double *rho, double *epsl, double *pres,
+                       double init, int *iters) {
+  double pmin = sqrt(conS2) - conD - conT;
+  pmin = dmax(pmin, P.patm);
+  double pold = dmax(init, pmin);
+  *iters = 0;
+  if (!isfinite(pold)) return C2P_FAIL;
+  double temp = conT + pold + conD;
+  double arg = temp * temp - conS2;
+  double sq = sqrt(arg);
+  double W = temp / sq;
+  *rho = conD / W;
+  *epsl = (sq - W * pold) / conD - 1.;
+  if (arg <= 0.0) return C2P_COLD;
+  int count;
+  double error = 2 * P.errormax;
+  for (count = 0; count <= P.countmax; count++) {
+    double peos, chi, kappa;
+    const int problem = eos_re(P, *rho, *epsl, &peos, &chi, &kappa);
+    const double f = pold - peos;
+    const double drhodp = conD * conS2 / (sq * temp * temp);
+    const double depsldp = pold * conS2 / (conD * arg * sq);
+    const double df = 1. - chi * drhodp - kappa * depsldp;
+    *pres = dmax(pmin, pold - f / df);
+    error = fabs(1. - pold / (*pres));
+    pold = *pres;
+    temp = conT + pold + conD;
+    arg = temp * temp - conS2;
+    sq = sqrt(arg);
+    W = temp / sq;
+    *rho = conD / W;
+    *epsl = (sq - W * pold) / conD - 1.;
+    *iters = count + 1;
+    if (*rho < P.fatm * P.rhoatm) return C2P_ATM;
+    if (problem) return C2P_ATM;
+    if (df == 0.) return C2P_NR;
+    if (error < P.errormax) break;
+  }
+  if (count >= P.countmax) return C2P_NR;
+  if (!isfinite(*pres) || !isfinite(*rho) || !isfinite(*epsl)) return C2P_COLD;
+  if (*epsl < 0.) return C2P_COLD;
+  const double v2 = conS2 / ((conT + conD + *pres) * (conT + conD + *pres));
+  if (v2 > P.vmax) return C2P_ATM;
+  return C2P_OK;
+}
+
+__device__ int try_cold(const C2PK &P, double conD, double conS2, double *conT, double *rho, double *epsl,
+                        double *pres, double init, int *iters) {
+  const double BIG = 1e+15;
+  double h, dhdrho, chi, arg, W, g, dg;
+  *iters = 0;
+  if (conD <= 0.) return C2P_ATM;
+  const double rhomin = P.rhoatm;
+  double rhold = dmax(init, rhomin);
+  if (!isfinite(rhold)) return C2P_FAIL;
+  int count;
+  double error = 2 * P.errormax;
+  for (count = 0; count <= P.countmax; count++) {
+    eos_cold(P, rhold, pres, epsl, &chi);
+    h = 1.0 + *epsl + *pres / rhold;
+    dhdrho = chi / rhold;
+    if (conD * h == 0.) arg = 1.0 + conS2 * BIG;
+    else arg = 1.0 + conS2 / ((conD * h) * (conD * h));
+    W = sqrt(arg);
+    g = W * rhold - conD;
+    dg = W - (rhold * conS2 * dhdrho) / (W * conD * conD * h * h * h);
+    *iters = count + 1;
+    if ((!isfinite(dg)) || (dg == 0.0)) return C2P_NR;
+    *rho = dmax(rhomin, rhold - g / dg);
+    error = fabs(1. - rhold / (*rho));
+    rhold = *rho;
+    if (error < P.errormax) break;
+  }
+  if (count >= P.countmax) return C2P_NR;
+  eos_cold(P, rhold, pres, epsl, &chi);
+  h = 1.0 + *epsl + *pres / rhold;
+  if (conD * h == 0.) arg = 1.0 + conS2 * BIG;
+  else arg = 1.0 + conS2 / ((conD * h) * (conD * h));
+  W = sqrt(arg);
+  if (*rho < P.fatm * P.rhoatm) return C2P_ATM;
+  *conT = W * W * (*rho) * h - *pres - conD;
+  const double v2 = conS2 / ((*conT + conD + *pres) * (*conT + conD + *pres));
+  if (v2 > P.vmax) return C2P_ATM;
+  return C2P_OK;
+}
+
+__device__ __forceinline__ double invg(double g11, double g12, double g13, double g22, double g23, double g33,
+                                       double *i11, double *i12, double *i13, double *i22, double *i23,
+                                       double *i33) {
+  const double c11 = g22 * g33 - g23 * g23;
+  const double c12 = g13 * g23 - g12 * g33;
+  const double c13 = g12 * g23 - g13 * g22;
+  const double c22 = g11 * g33 - g13 * g13;
+  const double c23 = g12 * g13 - g11 * g23;
+  const double c33 = g11 * g22 - g12 * g12;
+  const double det = g11 * c11 + g12 * c12 + g13 * c13;
+  *i11 = c11 / det;
+  *i12 = c12 / det;
+  *i13 = c13 / det;
+  *i22 = c22 / det;
+  *i23 = c23 / det;
+  *i33 = c33 / det;
+  return det;
+}
+
+// SoA views: cons (D, T, Sx, Sy, Sz), metric (xx xy xz yy yz zz),
+// prim (rho, eps, vx, vy, vz, p, v2, detg)
+struct C2PArrays {
+  double *cons[5];
+  const double *g[6];
+  double *prim[8];
+  int *status;
+  int *iters;
+};
+
+// hybrid p-root with cold static atmosphere (grhd_c2p_v2.c:303-460)
+__global__ void c2p_hybrid_kernel(C2PArrays A, C2PK P, long n) {
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const double gxx = A.g[0][q], gxy = A.g[1][q], gxz = A.g[2][q], gyy = A.g[3][q], gyz = A.g[4][q],
+                 gzz = A.g[5][q];
+    double gixx, gixy, gixz, giyy, giyz, gizz;
+    double detg = invg(gxx, gxy, gxz, gyy, gyz, gzz, &gixx, &gixy, &gixz, &giyy, &giyz, &gizz);
+    double SQRTdetg = sqrt(fabs(detg));
+    if (!isfinite(SQRTdetg)) {
+      detg = 1.;
+      SQRTdetg = 1.;
+    }
+    const double oo = 1. / SQRTdetg;
+    double conD = A.cons[0][q] * oo, conT = A.cons[1][q] * oo;
+    double conSx = A.cons[2][q] * oo, conSy = A.cons[3][q] * oo, conSz = A.cons[4][q] * oo;
+    double conSix = gixx * conSx + gixy * conSy + gixz * conSz;
+    double conSiy = gixy * conSx + giyy * conSy + giyz * conSz;
+    double conSiz = gixz * conSx + giyz * conSy + gizz * conSz;
+    double conS2 = conSx * conSix + conSy * conSiy + conSz * conSiz;
+    double SQRTconS2 = sqrt(fabs(conS2));
+    int result = C2P_OK;
+    if (!isfinite(conD) || !isfinite(conSx) || !isfinite(conSy) || !isfinite(conSz) || !isfinite(conT))
+      result = C2P_FAIL;
+    if (!isfinite(gxx) || !isfinite(gxy) || !isfinite(gxz) || !isfinite(gyy) || !isfinite(gyz) || !isfinite(gzz) ||
+        !isfinite(detg) || !isfinite(gixx) || !isfinite(gixy) || !isfinite(gixz) || !isfinite(giyy) ||
+        !isfinite(giyz) || !isfinite(gizz))
+      result = C2P_FAIL;
+    if (!isfinite(conSix) || !isfinite(conSiy) || !isfinite(conSiz) || !isfinite(conS2) || !isfinite(SQRTconS2)) {
+      conSx = conSix = 0.;
+      conSy = conSiy = 0.;
+      conSz = conSiz = 0.;
+      conS2 = SQRTconS2 = 0.;
+      result = C2P_FAIL;
+    }
+    double rho = 0, epsl = 0, pres = 0;
+    int it = 0, it2 = 0;
+    if (result != C2P_FAIL) result = try_hot(P, conD, conS2, conT, &rho, &epsl, &pres, A.prim[5][q], &it);
+    if (result == C2P_COLD) {
+      result = try_cold(P, conD, conS2, &conT, &rho, &epsl, &pres, A.prim[0][q], &it2);
+      A.cons[1][q] = SQRTdetg * conT;
+    }
+    int status = result;
+    if (result != C2P_OK && result != C2P_ATM) result = C2P_ATM;
+    if (result == C2P_ATM) {
+      pres = P.patm;
+      rho = P.rhoatm;
+      epsl = P.epslatm;
+      conD = P.rhoatm;
+      conSx = conSix = 0.;
+      conSy = conSiy = 0.;
+      conSz = conSiz = 0.;
+      conS2 = 0.;
+      conT = P.rhoatm * P.epslatm;
+      A.cons[0][q] = SQRTdetg * conD;
+      A.cons[2][q] = SQRTdetg * conSx;
+      A.cons[3][q] = SQRTdetg * conSy;
+      A.cons[4][q] = SQRTdetg * conSz;
+      A.cons[1][q] = SQRTdetg * conT;
+    }
+    const double ooDpT = 1.0 / (conD + pres + conT);
+    const double vx = conSix * ooDpT, vy = conSiy * ooDpT, vz = conSiz * ooDpT;
+    A.prim[0][q] = rho;
+    A.prim[1][q] = epsl;
+    A.prim[2][q] = vx;
+    A.prim[3][q] = vy;
+    A.prim[4][q] = vz;
+    A.prim[5][q] = pres;
+    const double vlx = gxx * vx + gxy * vy + gxz * vz;
+    const double vly = gxy * vx + gyy * vy + gyz * vz;
+    const double vlz = gxz * vx + gyz * vy + gzz * vz;
+    A.prim[6][q] = vlx * vx + vly * vy + vlz * vz;
+    A.prim[7][q] = detg;
+    A.status[q] = status;
+    A.iters[q] = it + it2;
+  }
+}
+
+// plain p-root (grhd_c2p.c:102-237); errorexit cases become FAIL / NR
+__global__ void c2p_plain_kernel(C2PArrays A, C2PK P, long n) {
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const double gxx = A.g[0][q], gxy = A.g[1][q], gxz = A.g[2][q], gyy = A.g[3][q], gyz = A.g[4][q],
+                 gzz = A.g[5][q];
+    double gixx, gixy, gixz, giyy, giyz, gizz;
+    const double detg = invg(gxx, gxy, gxz, gyy, gyz, gzz, &gixx, &gixy, &gixz, &giyy, &giyz, &gizz);
+    A.prim[7][q] = detg;
+    int status = C2P_OK, iters = 0;
+    if (detg <= 0. || !isfinite(detg)) {
+      A.status[q] = C2P_FAIL;
+      A.iters[q] = 0;
+      continue;
+    }
+    const double oo = 1. / sqrt(detg);
+    const double conD = A.cons[0][q] * oo, conT = A.cons[1][q] * oo;
+    const double conSx = A.cons[2][q] * oo, conSy = A.cons[3][q] * oo, conSz = A.cons[4][q] * oo;
+    const double conSix = gixx * conSx + gixy * conSy + gixz * conSz;
+    const double conSiy = gixy * conSx + giyy * conSy + giyz * conSz;
+    const double conSiz = gixz * conSx + giyz * conSy + gizz * conSz;
+    const double conS2 = conSx * conSix + conSy * conSiy + conSz * conSiz;
+    const double SQRTconS2 = sqrt(conS2);
+    if (!isfinite(conD) || !isfinite(conSx) || !isfinite(conSy) || !isfinite(conSz) || !isfinite(conT) ||
+        !isfinite(SQRTconS2)) {
+      A.status[q] = C2P_FAIL;
+      A.iters[q] = 0;
+      continue;
+    }
+    const double pmin = SQRTconS2 - conD - conT;
+    double pold = dmax(A.prim[5][q], pmin);
+    double temp = conT + pold + conD, arg = temp * temp - conS2, sq = sqrt(arg), W = temp / sq;
+    double rho = conD / W, eps = (sq - W * pold) / conD - 1.;
+    if (!isfinite(pold) || arg < 0.0) {
+      A.status[q] = C2P_FAIL;
+      A.iters[q] = 0;
+      continue;
+    }
+    double pnew = pold;
+    int count;
+    for (count = 0; count <= P.countmax; count++) {
+      double peos, chi, kappa;
+      eos_re(P, rho, eps, &peos, &chi, &kappa);
+      const double f = pold - peos;
+      const double drhodp = conD * conS2 / (sq * temp * temp);
+      const double depsldp = pold * conS2 / (conD * arg * sq);
+      const double df = 1. - chi * drhodp - kappa * depsldp;
+      if (df == 0.0) {
+        status = C2P_NR;
+        break;
+      }
+      pnew = dmax(pmin, pold - f / df);
+      const double error = fabs(1. - pold / pnew);
+      pold = pnew;
+      temp = conT + pold + conD;
+      arg = temp * temp - conS2;
+      sq = sqrt(arg);
+      W = temp / sq;
+      rho = conD / W;
+      eps = (sq - W * pold) / conD - 1.;
+      iters = count + 1;
+      if (error < P.errormax) break;
+    }
+    if (status == C2P_OK && count >= P.countmax) status = C2P_NR;
+    if (status == C2P_OK && eps < 0.) status = C2P_FAIL;
+    // grhd_c2p.c:224-226 divides each component
+    const double vx = conSix / (conD + pnew + conT), vy = conSiy / (conD + pnew + conT),
+                 vz = conSiz / (conD + pnew + conT);
+    A.prim[0][q] = rho;
+    A.prim[1][q] = eps;
+    A.prim[5][q] = pnew;
+    A.prim[2][q] = vx;
+    A.prim[3][q] = vy;
+    A.prim[4][q] = vz;
+    const double vlx = gxx * vx + gxy * vy + gxz * vz;
+    const double vly = gxy * vx + gyy * vy + gyz * vz;
+    const double vlz = gxz * vx + gyz * vy + gzz * vz;
+    A.prim[6][q] = vlx * vx + vly * vy + vlz * vz;
+    A.status[q] = status;
+    A.iters[q] = iters;
+  }
+}
+
+// Bracketed Newton-Raphson on the pressure: the safeguarded solver of this
+// build (no reference counterpart).  f(p) and f'(p) are those of
+// try_hot_part; the bracket [plo, phi] keeps the sign change (plo =
+// max(|S|-D-tau, 0), phi = (Gamma-1)(tau+D) since rho eps <= tau + D), a
+// Newton step leaving it is replaced by bisection, and the iteration stops
+// when |f| is within its rounding floor (sqrt(temp^2 - S^2) has a relative
+// error ~ eps_mach W^2), or the relative update / bracket is below
+// errormax, or after countmax iterations (NR).  guess <= 0: start at plo.
+__device__ int c2p_bracketed_pt(const C2PK &P, double conD, double conS2, double conT, double guess, double *rho,
+                                double *eps, double *pres, int *iters) {
+  double plo = sqrt(conS2) - conD - conT;
+  if (plo < 0.) plo = 0.;
+  double phi = (P.gamma - 1.) * (conT + conD);
+  if (!(phi > plo)) phi = plo + 1e-300;
+  double p = (guess > plo && guess < phi) ? guess : plo;
+  const double em = 2.220446049250313e-16;
+  int count;
+  *iters = 0;
+  for (count = 0; count < P.countmax; count++) {
+    const double temp = conT + p + conD;
+    const double arg = temp * temp - conS2;
+    const double sq = sqrt(arg);
+    const double W = temp / sq;
+    const double r = conD / W;
+    const double e = (sq - W * p) / conD - 1.;
+    double peos, chi, kappa;
+    eos_re(P, r, e, &peos, &chi, &kappa);
+    const double f = p - peos;
+    const double noise =
+        4. * em * (fabs(p) + fabs(peos) + fabs(chi) * r + fabs(kappa) * (W * W * sq + W * fabs(p)) / conD);
+    *iters = count + 1;
+    if (fabs(f) <= noise) break;
+    if (f < 0.) plo = p;
+    else phi = p;
+    const double drhodp = conD * conS2 / (sq * temp * temp);
+    const double depsldp = p * conS2 / (conD * arg * sq);
+    const double df = 1. - chi * drhodp - kappa * depsldp;
+    double pn = p - f / df;
+    if (!(pn >= plo && pn <= phi)) pn = 0.5 * (plo + phi);
+    const double step = fabs(pn - p);
+    p = pn;
+    if (step <= P.errormax * p || phi - plo <= P.errormax * p) break;
+  }
+  const double temp = conT + p + conD;
+  const double sq = sqrt(temp * temp - conS2);
+  const double W = temp / sq;
+  *rho = conD / W;
+  *eps = (sq - W * p) / conD - 1.;
+  *pres = p;
+  if (count >= P.countmax) return C2P_NR;
+  if (!isfinite(p) || !isfinite(*rho) || !isfinite(*eps)) return C2P_FAIL;
+  return C2P_OK;
+}
+
+// undensitise and raise S with the inverse metric: shared by the bracketed
+// and learned solvers
+struct Undens {
+  double D, T, S2, Six, Siy, Siz, detg;
+};
+__device__ __forceinline__ Undens undensitise(const C2PArrays &A, long q) {
+  Undens u;
+  double gixx, gixy, gixz, giyy, giyz, gizz;
+  u.detg = invg(A.g[0][q], A.g[1][q], A.g[2][q], A.g[3][q], A.g[4][q], A.g[5][q], &gixx, &gixy, &gixz, &giyy,
+                &giyz, &gizz);
+  const double oo = 1. / sqrt(u.detg);
+  u.D = A.cons[0][q] * oo;
+  u.T = A.cons[1][q] * oo;
+  const double Sx = A.cons[2][q] * oo, Sy = A.cons[3][q] * oo, Sz = A.cons[4][q] * oo;
+  u.Six = gixx * Sx + gixy * Sy + gixz * Sz;
+  u.Siy = gixy * Sx + giyy * Sy + giyz * Sz;
+  u.Siz = gixz * Sx + giyz * Sy + gizz * Sz;
+  u.S2 = Sx * u.Six + Sy * u.Siy + Sz * u.Siz;
+  return u;
+}
+
+__device__ __forceinline__ void store_prims(const C2PArrays &A, long q, const Undens &u, double rho, double eps,
+                                            double p) {
+  const double ooDpT = 1.0 / (u.D + p + u.T);
+  const double vx = u.Six * ooDpT, vy = u.Siy * ooDpT, vz = u.Siz * ooDpT;
+  A.prim[0][q] = rho;
+  A.prim[1][q] = eps;
+  A.prim[2][q] = vx;
+  A.prim[3][q] = vy;
+  A.prim[4][q] = vz;
+  A.prim[5][q] = p;
+  const double vlx = A.g[0][q] * vx + A.g[1][q] * vy + A.g[2][q] * vz;
+  const double vly = A.g[1][q] * vx + A.g[3][q] * vy + A.g[4][q] * vz;
+  const double vlz = A.g[2][q] * vx + A.g[4][q] * vy + A.g[5][q] * vz;
+  A.prim[6][q] = vlx * vx + vly * vy + vlz * vz;
+  A.prim[7][q] = u.detg;
+}
+
+__global__ void c2p_bracketed_kernel(C2PArrays A, C2PK P, long n) {
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const Undens u = undensitise(A, q);
+    double rho, eps, p;
+    int it;
+    const int st = c2p_bracketed_pt(P, u.D, u.S2, u.T, A.prim[5][q], &rho, &eps, &p, &it);
+    store_prims(A, q, u, rho, eps, p);
+    A.status[q] = st;
+    A.iters[q] = it;
+  }
+}
+
+static C2PK make_c2pk(const bamhip_c2p_params *p) {
+  C2PK K;
+  K.eos = p->eos;
+  K.gamma = p->gamma;
+  K.K = p->K;
+  K.rhoatm = p->rhoatm;
+  K.fatm = p->fatm;
+  K.epslatm = p->epslatm;
+  K.patm = p->patm;
+  K.vmax = p->vmax;
+  K.errormax = p->errormax;
+  K.countmax = p->countmax;
+  return K;
+}
+
+int c2p_launch(const bamhip_c2p_params *par, int method, long n, C2PArrays &A) {
+  if (!par || n < 0) return set_error(BAMHIP_EINVAL, "bamhip_c2p: bad arguments");
+  if (par->eos != 0 && par->eos != 1) return set_error(BAMHIP_EINVAL, "bamhip_c2p: eos=%d (0 ideal, 1 poly)", par->eos);
+  if (!(par->gamma > 1.0)) return set_error(BAMHIP_EINVAL, "bamhip_c2p: gamma must be > 1");
+  if (par->countmax < 1) return set_error(BAMHIP_EINVAL, "bamhip_c2p: countmax < 1");
+  if (n == 0) return BAMHIP_OK;
+  const C2PK K = make_c2pk(par);
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  TimerScope t("c2p");
+  if (method == BAMHIP_C2P_HYBRID)
+    c2p_hybrid_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(A, K, n);
+  else if (method == BAMHIP_C2P_PLAIN)
+    c2p_plain_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(A, K, n);
+  else if (method == BAMHIP_C2P_BRACKETED)
+    c2p_bracketed_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(A, K, n);
+  else
+    return set_error(BAMHIP_EINVAL, "bamhip_c2p: unknown method %d", method);
+  BH_HIP(hipGetLastError());
+  return BAMHIP_OK;
+}
+
+}  // namespace bamhip
+
+using namespace bamhip;
+
+extern "C" int bamhip_c2p_batch(const bamhip_c2p_params *par, int method, long n, double *cons, const double *metric,
+                                double *prim, int *status, int *iters) {
+  if (!cons || !metric || !prim || !status || !iters) return set_error(BAMHIP_EINVAL, "bamhip_c2p_batch: NULL array");
+  C2PArrays A;
+  for (int k = 0; k < 5; k++) A.cons[k] = cons + k * n;
+  for (int k = 0; k < 6; k++) A.g[k] = metric + k * n;
+  for (int k = 0; k < 8; k++) A.prim[k] = prim + k * n;
+  A.status = status;
+  A.iters = iters;
+  return c2p_launch(par, method, n, A);
+}
+
+extern "C" int bamhip_grhd_c2p(bamhip_level *lev, const bamhip_c2p_params *par, int method, const int *cons5,
+                               const int *metric6, const int *prim8, int *status, int *iters) {
+  if (!lev) return set_error(BAMHIP_EINVAL, "bamhip_grhd_c2p: NULL level");
+  if (!status || !iters) return set_error(BAMHIP_EINVAL, "bamhip_grhd_c2p: status/iters buffers are required");
+  C2PArrays A;
+  double *tmp[8];
+  int e;
+  if ((e = gather_ptrs(lev, cons5, 5, tmp, "bamhip_grhd_c2p(cons)"))) return e;
+  for (int k = 0; k < 5; k++) A.cons[k] = tmp[k];
+  if ((e = gather_ptrs(lev, metric6, 6, tmp, "bamhip_grhd_c2p(metric)"))) return e;
+  for (int k = 0; k < 6; k++) A.g[k] = tmp[k];
+  if ((e = gather_ptrs(lev, prim8, 8, tmp, "bamhip_grhd_c2p(prim)"))) return e;
+  for (int k = 0; k < 8; k++) A.prim[k] = tmp[k];
+  A.status = status;
+  A.iters = iters;
+  return c2p_launch(par, method, lev->npoints, A);
+}

@@ -155,6 +155,61 @@ __global__ void periodic_kernel(Ptrs25 u, int nv, DevBox B, int d, int g) {
   p[base + (nd - g + l) * st[d]] = p[base + (g + l) * st[d]];
 }
 
+// All periodic ghost points of a box in one pass: a point within g layers
+// of a periodic face takes the value of its image (i -> i +- (n - 2g)) in
+// every periodic direction at once, which equals the reference's sequence
+// of x, y, z slab copies (edges and corners included).  One thread per
+// ghost point and variable; the ghost set is enumerated as 6 slabs without
+// overlap: x slabs (full yz planes), y slabs (interior x), z slabs
+// (interior x and y).
+__global__ void periodic_all_kernel(Ptrs25 u, DevBox B, int g, int px, int py, int pz) {
+  const int M = B.m, N = B.n, O = B.o;
+  const int ix = px ? M - 2 * g : M, iy = py ? N - 2 * g : N;   // extent of the non-ghost part
+  const long sx = px ? (long)g * N * O : 0;                     // points per x slab
+  const long sy = py ? (long)g * ix * O : 0;
+  const long sz = pz ? (long)g * ix * iy : 0;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long r = t;
+  int i, j, k;
+  if (r < 2 * sx) {
+    const int side = (int)(r / sx);
+    r %= sx;
+    const int l = (int)(r % g);
+    r /= g;
+    j = (int)(r % N);
+    k = (int)(r / N);
+    i = side ? M - g + l : l;
+  } else if ((r -= 2 * sx) < 2 * sy) {
+    const int side = (int)(r / sy);
+    r %= sy;
+    const int l = (int)(r % g);
+    r /= g;
+    i = (px ? g : 0) + (int)(r % ix);
+    k = (int)(r / ix);
+    j = side ? N - g + l : l;
+  } else if ((r -= 2 * sy) < 2 * sz) {
+    const int side = (int)(r / sz);
+    r %= sz;
+    const int l = (int)(r % g);
+    r /= g;
+    i = (px ? g : 0) + (int)(r % ix);
+    j = (py ? g : 0) + (int)(r / ix);
+    k = side ? O - g + l : l;
+  } else {
+    return;
+  }
+  auto img = [g](int a, int n, int per) {
+    if (!per) return a;
+    if (a < g) return a + (n - 2 * g);
+    if (a >= n - g) return a - (n - 2 * g);
+    return a;
+  };
+  const long dst = B.noffset + i + (long)M * (j + (long)N * k);
+  const long src = B.noffset + img(i, M, px) + (long)M * (img(j, N, py) + (long)N * img(k, O, pz));
+  double *p = u.p[blockIdx.y];
+  p[dst] = p[src];
+}
+
 // ---------------------------------------------------------------- algcon
 // z4_algcon.m:14-48 on all points
 __global__ void algcon_kernel(Ptrs25 g6, Ptrs25 A6, long n, int subtractA, int normalizedetg) {
@@ -392,14 +447,19 @@ int bamhip_z4c_set_boundary(bamhip_level *lev, const bamhip_z4c_params *par, con
     if (anyper) {
       if (pgh < 1) return set_error(BAMHIP_EINVAL, "bamhip_z4c_set_boundary: periodic faces need pgh >= 1");
       const int dims[3] = {B.m, B.n, B.o};
+      int per[3];
       for (int d = 0; d < 3; d++) {
-        if (B.bflag[2 * d] != BAMHIP_PERBOUND || B.bflag[2 * d + 1] != BAMHIP_PERBOUND) continue;
-        if (dims[d] < 3 * pgh)
+        per[d] = (B.bflag[2 * d] == BAMHIP_PERBOUND && B.bflag[2 * d + 1] == BAMHIP_PERBOUND);
+        if (per[d] && dims[d] < 3 * pgh)
           return set_error(BAMHIP_EINVAL, "bamhip_z4c_set_boundary: periodic direction %d has %d < 3*%d points", d,
                            dims[d], pgh);
-        const long plane = (long)dims[(d + 1) % 3] * dims[(d + 2) % 3];
-        dim3 grid((unsigned)((plane * pgh + 255) / 256), 25, 1);
-        periodic_kernel<<<grid, 256, 0, stream()>>>(N, 25, B, d, pgh);
+      }
+      const long ix = per[0] ? B.m - 2 * pgh : B.m, iy = per[1] ? B.n - 2 * pgh : B.n;
+      const long total = 2 * ((per[0] ? (long)pgh * B.n * B.o : 0) + (per[1] ? (long)pgh * ix * B.o : 0) +
+                              (per[2] ? (long)pgh * ix * iy : 0));
+      if (total > 0) {
+        dim3 grid((unsigned)((total + 255) / 256), 25, 1);
+        periodic_all_kernel<<<grid, 256, 0, stream()>>>(N, B, pgh, per[0], per[1], per[2]);
         BH_HIP(hipGetLastError());
       }
     }

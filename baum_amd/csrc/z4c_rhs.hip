@@ -608,18 +608,46 @@ __device__ __forceinline__ void t_hess(const double *T, const Z4cGeo &G, double 
   d[5] = t_d2<2>(T, G);
 }
 
-__device__ __forceinline__ void stage(double *tile, const double *__restrict__ f, const Z4cGeo &G, int i0, int j0,
-                                      int k0, int tid) {
-  __syncthreads();
-  for (int idx = tid; idx < TILE; idx += NTHR) {
-    const int x = idx % SX, y = (idx / SX) % SY, z = idx / (SX * SY);
-    const int gi = i0 - HALO + x, gj = j0 - HALO + y, gk = k0 - HALO + z;
-    double v = 0.0;
-    if (gi < G.m && gj < G.n && gk < G.o) v = f[G.noffset + gi + G.sj * gj + G.sk * gk];
-    tile[idx] = v;
+// Staging of one field into the LDS tile, software-pipelined: the global
+// loads of the next field are issued into registers (NLOAD per thread, all
+// in flight together) before the current field is computed, and written to
+// LDS between two barriers at the start of the next stage.
+constexpr int NLOAD = (TILE + NTHR - 1) / NTHR;
+// staging order of the 25 fields: gt_ij, Gamma^i, chi, alpha, Khat, Theta,
+// At_ij, beta^i, B^i
+__device__ constexpr int STAGE_ORDER[25] = {0,  1,  2,  3,  4,  5,  14, 15, 16, 6,  18, 13, 17,
+                                            7,  8,  9,  10, 11, 12, 19, 20, 21, 22, 23, 24};
+
+struct Stager {
+  int goff[NLOAD];
+  unsigned ok;      // bit k: slot k lies inside the tile and the box
+  long cbase;       // linear index of the tile corner (i0-3, j0-3, k0-3)
+  double pf[NLOAD];
+  __device__ Stager(const Z4cGeo &G, int i0, int j0, int k0, int tid) {
+    cbase = G.noffset + (i0 - HALO) + G.sj * (j0 - HALO) + G.sk * (k0 - HALO);
+    ok = 0;
+#pragma unroll
+    for (int k = 0; k < NLOAD; k++) {
+      const int idx = tid + k * NTHR;
+      const int x = idx % SX, y = (idx / SX) % SY, z = idx / (SX * SY);
+      const bool in = idx < TILE && (i0 - HALO + x) < G.m && (j0 - HALO + y) < G.n && (k0 - HALO + z) < G.o;
+      goff[k] = in ? (int)(x + G.sj * y + G.sk * z) : 0;
+      ok |= in ? (1u << k) : 0u;
+    }
   }
-  __syncthreads();
-}
+  __device__ __forceinline__ void load(const double *__restrict__ f) {
+    const double *fb = f + cbase;
+#pragma unroll
+    for (int k = 0; k < NLOAD; k++) pf[k] = (ok >> k & 1u) ? fb[goff[k]] : 0.0;
+  }
+  __device__ __forceinline__ void commit(double *tile, int tid) {
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NLOAD; k++)
+      if (tid + k * NTHR < TILE) tile[tid + k * NTHR] = pf[k];
+    __syncthreads();
+  }
+};
 
 // Pin a value in registers at this point of the program: an empty volatile
 // asm that reads and writes it.  Placed after each staged field, it keeps the
@@ -670,6 +698,13 @@ __global__ void __launch_bounds__(NTHR) z4c_rhs_bulk(Z4cPtrs P, Z4cGeo G, Z4cK K
     Theta = u[17][q];
     alpha = u[18][q];
   }
+  Stager st(G, i0, j0, k0, tid);
+  st.load(u[STAGE_ORDER[0]]);
+  // stage s: commit the prefetched field, then prefetch field s+1
+  auto advance = [&](int sidx) {
+    st.commit(tile, tid);
+    if (sidx + 1 < 25) st.load(u[STAGE_ORDER[sidx + 1]]);
+  };
   double pw[3], mw[3];
 #pragma unroll
   for (int a = 0; a < 3; a++) {
@@ -706,7 +741,7 @@ __global__ void __launch_bounds__(NTHR) z4c_rhs_bulk(Z4cPtrs P, Z4cGeo G, Z4cK K
   double dg[3][6];
 static_for<6>([&](auto I_) {
     constexpr int ab = decltype(I_)::value;
-    stage(tile, u[ab], G, i0, j0, k0, tid);
+    advance(ab);
     dg[0][ab] = t_d1<0>(T, G);
     dg[1][ab] = t_d1<1>(T, G);
     dg[2][ab] = t_d1<2>(T, G);
@@ -773,7 +808,7 @@ static_for<6>([&](auto I_) {
   double advG[3], koG[3];
 static_for<3>([&](auto I_) {
     constexpr int c = decltype(I_)::value;
-    stage(tile, u[14 + c], G, i0, j0, k0, tid);
+    advance(6 + c);
     double dG[3];
     t_grad(T, G, dG);
 #pragma unroll
@@ -794,7 +829,7 @@ static_for<3>([&](auto I_) {
   const double oochig = 1.0 / chig;
   double dchi[3], df[3];
   {
-    stage(tile, u[6], G, i0, j0, k0, tid);
+    advance(9);
     t_grad(T, G, dchi);
     double h6[6];
     t_hess(T, G, h6);
@@ -828,7 +863,7 @@ static_for<3>([&](auto I_) {
   // ---- lapse
   double da[3], cdda[6];
   {
-    stage(tile, u[18], G, i0, j0, k0, tid);
+    advance(10);
     t_grad(T, G, da);
     double h6[6];
     t_hess(T, G, h6);
@@ -851,14 +886,14 @@ static_for<3>([&](auto I_) {
   // ---- Khat and Theta: 2 dKhat + dTheta
   double dKT[3], koT;
   {
-    stage(tile, u[13], G, i0, j0, k0, tid);
+    advance(11);
     dKT[0] = 2.0 * t_d1<0>(T, G);
     dKT[1] = 2.0 * t_d1<1>(T, G);
     dKT[2] = 2.0 * t_d1<2>(T, G);
     rhs[13] = t_adv(T, pw, mw) + eps * t_ko(T, G);
     pin(dKT);
     pin(rhs[13]);
-    stage(tile, u[17], G, i0, j0, k0, tid);
+    advance(12);
     dKT[0] += t_d1<0>(T, G);
     dKT[1] += t_d1<1>(T, G);
     dKT[2] += t_d1<2>(T, G);
@@ -872,7 +907,7 @@ static_for<3>([&](auto I_) {
   // ---- trace-free extrinsic curvature: advection and dissipation only
 static_for<6>([&](auto I_) {
     constexpr int ab = decltype(I_)::value;
-    stage(tile, u[7 + ab], G, i0, j0, k0, tid);
+    advance(13 + ab);
     rhs[7 + ab] = t_adv(T, pw, mw) + eps * t_ko(T, G);
     pin(rhs[7 + ab]);
   });
@@ -881,7 +916,7 @@ static_for<6>([&](auto I_) {
   double db[3][3], pG[3], ootd[3];
 static_for<3>([&](auto I_) {
     constexpr int c = decltype(I_)::value;
-    stage(tile, u[19 + c], G, i0, j0, k0, tid);
+    advance(19 + c);
     db[0][c] = t_d1<0>(T, G);
     db[1][c] = t_d1<1>(T, G);
     db[2][c] = t_d1<2>(T, G);
@@ -901,7 +936,7 @@ static_for<3>([&](auto I_) {
   // ---- B: advection and dissipation only
 static_for<3>([&](auto I_) {
     constexpr int c = decltype(I_)::value;
-    stage(tile, u[22 + c], G, i0, j0, k0, tid);
+    advance(22 + c);
     rhs[22 + c] = (K.withB != 0.0 ? K.withBadv * t_adv(T, pw, mw) : 0.0) + eps * t_ko(T, G);
     pin(rhs[22 + c]);
   });

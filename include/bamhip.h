@@ -163,6 +163,45 @@ int bamhip_z4c_algcon(bamhip_level *lev, const int *g, const int *A, int subtrac
 int bamhip_z4c_ham_l2(bamhip_level *lev, const int *u, double chipsipower, int ham,
                       double *l2);
 
+/* --------------------------------------------------------------- con2prim */
+/* EOS and atmosphere snapshot read by the reference c2p (GRHD.ATM_*,
+ * HRSC_VMAX, grhd_C2P_NewtonRaphsonTR/NR; src/projects/grhd/grhd_atm.c:55-87,
+ * bam_grhd.c:46-62, eos_analytic.c:52-72,423-434) */
+typedef struct {
+  int eos;          /* 0: ideal gas (cold part = polytrope), 1: polytrope */
+  double gamma, K;  /* Gamma (and K of the polytrope) */
+  double rhoatm, fatm, epslatm, patm;
+  double vmax;      /* grhd_vmax, compared with v^2 as in the reference */
+  double errormax;  /* grhd_C2P_NewtonRaphsonTR */
+  int countmax;     /* grhd_C2P_NewtonRaphsonNR */
+} bamhip_c2p_params;
+
+/* methods: the production hybrid p-root with cold static atmosphere
+ * (grhd_c2p_proot_ColdStaticAtm_hybrid, grhd_c2p_v2.c:265), the plain
+ * p-root without atmosphere (grhd_c2p_proot, grhd_c2p.c:102), and this
+ * library's bracketed p-root (Newton safeguarded by bisection, stopping at
+ * the rounding floor of the residual; no reference counterpart), which
+ * converges where the reference's relative-change test cannot be met */
+enum { BAMHIP_C2P_HYBRID = 0, BAMHIP_C2P_PLAIN = 1, BAMHIP_C2P_BRACKETED = 2 };
+/* status per point: the reference's codes (grhd_c2p_v2.c:14-20) */
+enum { BAMHIP_C2P_COLD = 0, BAMHIP_C2P_ATM = 1, BAMHIP_C2P_OK = 2, BAMHIP_C2P_FAIL = 3, BAMHIP_C2P_NR = 4 };
+
+/* Batched c2p on flat device arrays of n points (SoA):
+ * cons[5n] = densitised D, tau, S_x, S_y, S_z (rewritten on COLD/ATM, as the
+ * reference does), metric[6n] = gamma_ij (xx xy xz yy yz zz),
+ * prim[8n] = rho, eps, v^x, v^y, v^z, p, v^2, detg (on entry p and rho hold
+ * the previous values used as initial guesses), status[n], iters[n]. */
+int bamhip_c2p_batch(const bamhip_c2p_params *par, int method, long n, double *cons,
+                     const double *metric, double *prim, int *status, int *iters);
+
+/* Replaces MATTER.c2p(ucur, primVars) (src/projects/matter/bam_matter.h:49,
+ * called at matter.c:317,411) on a level: cons5 / metric6 / prim8 are the
+ * variable indices of the conserved variables, of adm_gxx.. and of the
+ * primVars list (rho, epsl, vx, vy, vz, p, v2, detg; grhd.c:46-53).
+ * status / iters: device arrays of npoints ints. */
+int bamhip_grhd_c2p(bamhip_level *lev, const bamhip_c2p_params *par, int method, const int *cons5,
+                    const int *metric6, const int *prim8, int *status, int *iters);
+
 /* ----------------------------------------------------------------- timers */
 /* accumulated device time per kernel family (BAM's timer_start/timer_stop
  * keys, src/main/bampi/timer.c:60-104), measured with HIP events on the

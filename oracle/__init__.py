@@ -80,6 +80,7 @@ def lib():
         L.ob_prolong_1d.argtypes = [C.c_int, C.c_int, DP, DP]
         L.ob_c2p_hybrid.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP, IP, IP]
         L.ob_c2p_proot.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP, IP, IP]
+        L.ob_c2p_bracketed.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP, IP, IP]
         L.ob_p2c.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP]
         L.ob_try_hot.restype = C.c_int
         L.ob_try_hot.argtypes = [C.POINTER(C2PParams), C.c_double, C.c_double, C.c_double,
@@ -256,7 +257,8 @@ def c2p(P, cons, metric, pguess, rhoguess=None, method="hybrid"):
         prim[0] = rhoguess
     st = np.zeros(n, dtype=np.int32)
     it = np.zeros(n, dtype=np.int32)
-    fn = lib().ob_c2p_hybrid if method == "hybrid" else lib().ob_c2p_proot
+    fn = {"hybrid": lib().ob_c2p_hybrid, "plain": lib().ob_c2p_proot,
+          "bracketed": lib().ob_c2p_bracketed}[method]
     fn(C.byref(P), n, cons.ctypes.data_as(DP), metric.ctypes.data_as(DP), prim.ctypes.data_as(DP),
        st.ctypes.data_as(IP), it.ctypes.data_as(IP))
     return prim, st, it, cons

@@ -349,3 +349,96 @@ void ob_c2p_proot(const ob_c2p_params *P, long n, const double *cons, const doub
     prim[6 * n + q] = vlx * vx + vly * vy + vlz * vz;
   }
 }
+
+/* Bracketed Newton-Raphson on the pressure (this build's safeguarded solver;
+ * no reference counterpart).  Same f(p) = p - P(rho(p), eps(p)) and
+ * derivative as try_hot_part, with a bracket [plo, phi] that always holds
+ * the sign change: plo = max(|S|-D-tau, 0), phi = (Gamma-1)(tau+D) for the
+ * ideal gas (rho eps <= tau + D).  A Newton step leaving the bracket is
+ * replaced by bisection.  Stops when the relative update or the relative
+ * bracket width is below errormax, or when |f| is within its rounding floor
+ * (4 eps_mach times the magnitudes entering f, amplified by W^2 where
+ * sqrt(temp^2 - S^2) cancels, so that cold fast flows, whose
+ * pressure the conserved variables fix only to ~1e-8, do not iterate on
+ * noise), or after countmax iterations (status NR).
+ * guess <= 0 starts from the bracket's lower end. */
+int ob_c2p_bracketed_pt(const ob_c2p_params *P, double conD, double conS2, double conT, double guess,
+                        double *rho, double *eps, double *pres, int *iters)
+{
+  double plo = sqrt(conS2) - conD - conT;
+  if (plo < 0.) plo = 0.;
+  double phi = (P->gamma - 1.) * (conT + conD);
+  if (!(phi > plo)) phi = plo + 1e-300;
+  double p = (guess > plo && guess < phi) ? guess : plo;
+  const double em = 2.220446049250313e-16;
+  int count;
+  *iters = 0;
+  for (count = 0; count < P->countmax; count++) {
+    const double temp = conT + p + conD;
+    const double arg = temp * temp - conS2;
+    const double sq = sqrt(arg);
+    const double W = temp / sq;
+    const double r = conD / W;
+    const double e = (sq - W * p) / conD - 1.;
+    double peos, chi, kappa;
+    eos_re(P, r, e, &peos, &chi, &kappa);
+    const double f = p - peos;
+    /* rounding floor of f: sq = sqrt(temp^2 - S^2) carries a relative error
+       ~ eps_mach W^2 (temp^2 / arg = W^2), which enters eps through sq / D */
+    const double noise = 4. * em * (fabs(p) + fabs(peos) + fabs(chi) * r + fabs(kappa) * (W * W * sq + W * fabs(p)) / conD);
+    *iters = count + 1;
+    if (fabs(f) <= noise) break;
+    if (f < 0.) plo = p; else phi = p;
+    const double drhodp = conD * conS2 / (sq * temp * temp);
+    const double depsldp = p * conS2 / (conD * arg * sq);
+    const double df = 1. - chi * drhodp - kappa * depsldp;
+    double pn = p - f / df;
+    if (!(pn >= plo && pn <= phi)) pn = 0.5 * (plo + phi);
+    const double step = fabs(pn - p);
+    p = pn;
+    if (step <= P->errormax * p || phi - plo <= P->errormax * p) break;
+  }
+  const double temp = conT + p + conD;
+  const double sq = sqrt(temp * temp - conS2);
+  const double W = temp / sq;
+  *rho = conD / W;
+  *eps = (sq - W * p) / conD - 1.;
+  *pres = p;
+  if (count >= P->countmax) return OB_C2P_NR;
+  if (!isfinite(p) || !isfinite(*rho) || !isfinite(*eps)) return OB_C2P_FAIL;
+  return OB_C2P_OK;
+}
+
+void ob_c2p_bracketed(const ob_c2p_params *P, long n, const double *cons, const double *metric,
+                      double *prim, int *status, int *iters)
+{
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (long q = 0; q < n; q++) {
+    const double *gm = metric;
+    double gixx, gixy, gixz, giyy, giyz, gizz;
+    const double detg = ob_invg(gm[q], gm[n + q], gm[2 * n + q], gm[3 * n + q], gm[4 * n + q], gm[5 * n + q],
+                                &gixx, &gixy, &gixz, &giyy, &giyz, &gizz);
+    prim[7 * n + q] = detg;
+    const double oo = 1. / sqrt(detg);
+    const double conD = cons[q] * oo, conT = cons[n + q] * oo;
+    const double conSx = cons[2 * n + q] * oo, conSy = cons[3 * n + q] * oo, conSz = cons[4 * n + q] * oo;
+    const double conSix = gixx * conSx + gixy * conSy + gixz * conSz;
+    const double conSiy = gixy * conSx + giyy * conSy + giyz * conSz;
+    const double conSiz = gixz * conSx + giyz * conSy + gizz * conSz;
+    const double conS2 = conSx * conSix + conSy * conSiy + conSz * conSiz;
+    double rho, eps, p;
+    status[q] = ob_c2p_bracketed_pt(P, conD, conS2, conT, prim[5 * n + q], &rho, &eps, &p, &iters[q]);
+    const double ooDpT = 1.0 / (conD + p + conT);
+    const double vx = conSix * ooDpT, vy = conSiy * ooDpT, vz = conSiz * ooDpT;
+    prim[q] = rho;
+    prim[n + q] = eps;
+    prim[2 * n + q] = vx;
+    prim[3 * n + q] = vy;
+    prim[4 * n + q] = vz;
+    prim[5 * n + q] = p;
+    const double vlx = gm[q] * vx + gm[n + q] * vy + gm[2 * n + q] * vz;
+    const double vly = gm[n + q] * vx + gm[3 * n + q] * vy + gm[4 * n + q] * vz;
+    const double vlz = gm[2 * n + q] * vx + gm[4 * n + q] * vy + gm[5 * n + q] * vz;
+    prim[6 * n + q] = vlx * vx + vly * vy + vlz * vz;
+  }
+}

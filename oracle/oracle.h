@@ -134,6 +134,13 @@ void ob_p2c(const ob_c2p_params *P, long n, const double *prim5, const double *m
  * errorexit cases become status FAIL (bad input) or NR (no convergence) */
 void ob_c2p_proot(const ob_c2p_params *P, long n, const double *cons, const double *metric,
                   double *prim, int *status, int *iters);
+/* bracketed Newton on the pressure with a rounding-floor stop (no reference
+ * counterpart: the safeguarded solver of this build); prim[5n..] holds the
+ * initial guess (<= 0: start at the bracket's lower end) */
+int ob_c2p_bracketed_pt(const ob_c2p_params *P, double D, double S2, double T, double guess,
+                        double *rho, double *eps, double *p, int *iters);
+void ob_c2p_bracketed(const ob_c2p_params *P, long n, const double *cons, const double *metric,
+                      double *prim, int *status, int *iters);
 
 #ifdef __cplusplus
 }
