@@ -78,6 +78,20 @@ def lib():
         L.ob_z4c_ham_l2.argtypes = [C.POINTER(Box), C.POINTER(DP), C.c_double, DP]
         L.ob_lagrange_coeffs.argtypes = [C.c_int, C.c_double, DP]
         L.ob_prolong_1d.argtypes = [C.c_int, C.c_int, DP, DP]
+        L.ob_prolong_box.argtypes = [C.c_int, IP, IP, IP, DP, DP, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.ob_restrict_box.argtypes = [C.c_int, IP, IP, DP, DP, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.ob_rp_flags.argtypes = [C.c_int, C.POINTER(Box), C.c_int, C.POINTER(Box), IP, C.c_int, DP, DP]
+        L.ob_prolong_region.restype = C.c_int
+        L.ob_prolong_region.argtypes = [C.POINTER(Box), C.POINTER(Box), DP, C.c_int, C.c_int, C.c_int,
+                                        IP, IP, IP]
+        L.ob_restrict_region.restype = C.c_int
+        L.ob_restrict_region.argtypes = [C.POINTER(Box), C.POINTER(Box), DP, C.c_int, C.c_int, IP, IP]
+        L.ob_prolong_level.restype = C.c_int
+        L.ob_prolong_level.argtypes = [C.c_int, C.POINTER(Box), C.c_int, C.POINTER(Box), IP, DP, C.c_int,
+                                       C.c_int, C.c_int, C.c_int, DP, DP]
+        L.ob_restrict_level.restype = C.c_int
+        L.ob_restrict_level.argtypes = [C.c_int, C.POINTER(Box), C.c_int, C.POINTER(Box), IP, DP, C.c_int,
+                                        C.c_int, C.c_int, DP, DP]
         L.ob_c2p_hybrid.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP, IP, IP]
         L.ob_c2p_proot.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP, IP, IP]
         L.ob_c2p_bracketed.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP, IP, IP]
@@ -217,6 +231,78 @@ def prolong_1d(order, uc):
     uf = np.zeros(2 * uc.size)
     lib().ob_prolong_1d(order, uc.size, uc.ctypes.data_as(DP), uf.ctypes.data_as(DP))
     return uf
+
+
+# ---------------------------------------------------------------- FMR coupling
+def _ivec(vals):
+    return (C.c_int * len(vals))(*vals)
+
+
+def _boxes(boxes):
+    return (Box * len(boxes))(*boxes)
+
+
+def level_points(boxes):
+    return sum(b.m * b.n * b.o for b in boxes)
+
+
+def prolong_box(order, cbox, fbox, fbox0, cv, fv):
+    """prolong_boxtobox_N on single boxes: cv (o,n,m) coarse, fv (o,n,m) fine
+    (updated in place); cbox/fbox/fbox0 are 6-int index boxes"""
+    assert cv.flags.c_contiguous and fv.flags.c_contiguous
+    lib().ob_prolong_box(order, _ivec(cbox), _ivec(fbox), _ivec(fbox0), cv.ctypes.data_as(DP),
+                         fv.ctypes.data_as(DP), cv.shape[2], cv.shape[2] * cv.shape[1], fv.shape[2],
+                         fv.shape[2] * fv.shape[1])
+
+
+def restrict_box(order, fbox, cbox, fv, cv):
+    assert cv.flags.c_contiguous and fv.flags.c_contiguous
+    lib().ob_restrict_box(order, _ivec(fbox), _ivec(cbox), fv.ctypes.data_as(DP), cv.ctypes.data_as(DP),
+                          fv.shape[2], fv.shape[2] * fv.shape[1], cv.shape[2], cv.shape[2] * cv.shape[1])
+
+
+def rp_flags(fboxes, cboxes, parent, nbuffer):
+    """setRPflags: returns (flagprolong over the fine level, flagrestrict over
+    the coarse level), each a flat array stored box after box"""
+    fp = np.zeros(level_points(fboxes))
+    fr = np.zeros(level_points(cboxes))
+    lib().ob_rp_flags(len(fboxes), _boxes(fboxes), len(cboxes), _boxes(cboxes), _ivec(parent), nbuffer,
+                      fp.ctypes.data_as(DP), fr.ctypes.data_as(DP))
+    return fp, fr
+
+
+def prolong_region(fbox, cbox, fp_box, nbuffer, zone, order):
+    """returns (n, ifbox, ifbox0, icbox)"""
+    a, b, c = (C.c_int * 6)(), (C.c_int * 6)(), (C.c_int * 6)()
+    fp_box = np.ascontiguousarray(fp_box, dtype=np.float64)
+    n = lib().ob_prolong_region(C.byref(fbox), C.byref(cbox), fp_box.ctypes.data_as(DP), nbuffer, zone,
+                                order, a, b, c)
+    return n, list(a), list(b), list(c)
+
+
+def restrict_region(fbox, cbox, fr_box, nbuffer, order):
+    """returns (n, icbox, ifbox)"""
+    a, b = (C.c_int * 6)(), (C.c_int * 6)()
+    fr_box = np.ascontiguousarray(fr_box, dtype=np.float64)
+    n = lib().ob_restrict_region(C.byref(fbox), C.byref(cbox), fr_box.ctypes.data_as(DP), nbuffer, order,
+                                 a, b)
+    return n, list(a), list(b)
+
+
+def prolong_level(fboxes, cboxes, parent, fp, nbuffer, zone, order, cv, fv):
+    """prolong_varlist_boxtobox_N for nvar variables: cv (nvar, ncoarse),
+    fv (nvar, nfine) updated in place; returns the status (0 or -(b+1))"""
+    assert cv.flags.c_contiguous and fv.flags.c_contiguous and cv.shape[0] == fv.shape[0]
+    return lib().ob_prolong_level(len(fboxes), _boxes(fboxes), len(cboxes), _boxes(cboxes), _ivec(parent),
+                                  np.ascontiguousarray(fp).ctypes.data_as(DP), nbuffer, zone, order,
+                                  cv.shape[0], cv.ctypes.data_as(DP), fv.ctypes.data_as(DP))
+
+
+def restrict_level(fboxes, cboxes, parent, fr, nbuffer, order, fv, cv):
+    assert cv.flags.c_contiguous and fv.flags.c_contiguous and cv.shape[0] == fv.shape[0]
+    return lib().ob_restrict_level(len(fboxes), _boxes(fboxes), len(cboxes), _boxes(cboxes), _ivec(parent),
+                                   np.ascontiguousarray(fr).ctypes.data_as(DP), nbuffer, order, cv.shape[0],
+                                   fv.ctypes.data_as(DP), cv.ctypes.data_as(DP))
 
 
 # ---------------------------------------------------------------- con2prim

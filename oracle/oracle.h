@@ -104,6 +104,22 @@ double ob_z4c_ham_l2(const ob_box *b, const double *const *u, double chipsipower
 void ob_lagrange_coeffs(int n, double x, double *c);
 void ob_prolong_1d(int order, int nc, const double *uc, double *uf);
 
+/* ---------------- FMR coupling (amr/flag.c, interpolate_boxtobox_N.c) ------ */
+void ob_prolong_box(int N, const int *cbox, const int *fbox, const int *fbox0, const double *cv, double *fv,
+                    int cdj, int cdk, int fdj, int fdk);
+void ob_restrict_box(int N, const int *fbox, const int *cbox, const double *fv, double *cv, int fdj, int fdk,
+                     int cdj, int cdk);
+void ob_rp_flags(int nf, const ob_box *fb, int nc, const ob_box *cb, const int *parent, int nbuffer, double *fp,
+                 double *fr);
+int ob_prolong_region(const ob_box *f, const ob_box *c, const double *fp, int nbuffer, int zone, int order,
+                      int *ifbox, int *ifbox0, int *icbox);
+int ob_restrict_region(const ob_box *f, const ob_box *c, const double *fr, int nbuffer, int order, int *icbox,
+                       int *ifbox);
+int ob_prolong_level(int nf, const ob_box *fb, int nc, const ob_box *cb, const int *parent, const double *fp,
+                     int nbuffer, int zone, int order, int nvar, const double *cv, double *fv);
+int ob_restrict_level(int nf, const ob_box *fb, int nc, const ob_box *cb, const int *parent, const double *fr,
+                      int nbuffer, int order, int nvar, const double *fv, double *cv);
+
 /* ---------------- con2prim (src/projects/grhd) ---------------------------- */
 typedef struct {
   int eos;               /* 0 ideal gas (cold part polytrope), 1 polytrope */
