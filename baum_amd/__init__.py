@@ -99,6 +99,12 @@ def lib():
     _sig(L, "bamhip_z4c_rk4_step", i, [vp, C.POINTER(Z4cParams), d, IP, IP, IP, IP, i, i])
     _sig(L, "bamhip_z4c_algcon", i, [vp, IP, IP, i, i])
     _sig(L, "bamhip_z4c_ham_l2", i, [vp, IP, d, i, DP])
+    _sig(L, "bamhip_set_rp_flags", i, [vp, vp, IP, i, i, i])
+    _sig(L, "bamhip_restrict_varlist", i, [vp, vp, i, IP, IP, i, i])
+    _sig(L, "bamhip_prolong_varlist", i, [vp, vp, i, IP, IP, i, i])
+    _sig(L, "bamhip_restrict_prolong_varlist", i, [vp, vp, i, IP, IP, i, i])
+    _sig(L, "bamhip_bo_interp_begin", i, [vp, i, i, i, IP, IP, IP, IP, IP, IP])
+    _sig(L, "bamhip_bo_interp_end", i, [vp, i, i, i, IP, IP])
     _sig(L, "bamhip_timer_enable", i, [i])
     _sig(L, "bamhip_timer_reset", i, [])
     _sig(L, "bamhip_timer_get", i, [C.c_char_p, DP, C.POINTER(C.c_long)])
@@ -282,6 +288,40 @@ class Level:
         out = C.c_double()
         check(lib().bamhip_z4c_ham_l2(self.h, _iarr(u), chipsipower, ham, C.byref(out)), "bamhip_z4c_ham_l2")
         return out.value
+
+    # -- Berger-Oliger time interpolation (evolve.c:298-340, 367-380)
+    def bo_interp_begin(self, timeinterpolation, iteration, upp, uc, up=None, up2=None, up3=None, up4=None):
+        opt = [_iarr(x) if x is not None else None for x in (up, up2, up3, up4)]
+        check(lib().bamhip_bo_interp_begin(self.h, timeinterpolation, iteration, len(upp), _iarr(upp), _iarr(uc),
+                                           *opt), "bamhip_bo_interp_begin")
+
+    def bo_interp_end(self, timeinterpolation, iteration, upp, uc):
+        check(lib().bamhip_bo_interp_end(self.h, timeinterpolation, iteration, len(upp), _iarr(upp), _iarr(uc)),
+              "bamhip_bo_interp_end")
+
+
+# ---------------------------------------------------------------- FMR coupling
+def set_rp_flags(lc, lf, parent, nbuffer, fp, fr):
+    """setRPflags(lc, lf): links lf to lc (fine box b in coarse box parent[b])"""
+    check(lib().bamhip_set_rp_flags(lc.h, lf.h, _iarr(parent), nbuffer, fp, fr), "bamhip_set_rp_flags")
+
+
+def restrict_varlist(lf, lc, uf, uc, nbuffer=3, order=4):
+    assert len(uf) == len(uc)
+    check(lib().bamhip_restrict_varlist(lf.h, lc.h, len(uf), _iarr(uf), _iarr(uc), nbuffer, order),
+          "bamhip_restrict_varlist")
+
+
+def prolong_varlist(lc, lf, uc, uf, nbuffer=3, order=4):
+    assert len(uf) == len(uc)
+    check(lib().bamhip_prolong_varlist(lc.h, lf.h, len(uc), _iarr(uc), _iarr(uf), nbuffer, order),
+          "bamhip_prolong_varlist")
+
+
+def restrict_prolong_varlist(lc, lf, uc, uf, nbuffer=3, order=4):
+    assert len(uf) == len(uc)
+    check(lib().bamhip_restrict_prolong_varlist(lc.h, lf.h, len(uc), _iarr(uc), _iarr(uf), nbuffer, order),
+          "bamhip_restrict_prolong_varlist")
 
 
 # ---------------------------------------------------------------- con2prim

@@ -33,6 +33,30 @@ struct DevBox {
   int bflag[6];
 };
 
+// Index regions of one fine box for the FMR transfers, derived from the
+// flag fields as prolong_varlist_boxtobox_N / restrict_varlist_boxtobox_N
+// derive them (n == 0: nothing to do for this box).
+struct ProlongRegion {
+  int n;
+  int ifbox[6], ifbox0[6], icbox[6];
+};
+struct RestrictRegion {
+  int n;
+  int icbox[6], ifbox[6];
+};
+
+// parent relation of a fine level (tL::prlocal, tB::pr) and the regions
+// recorded by bamhip_set_rp_flags
+struct FmrLink {
+  struct bamhip_level *lc = nullptr;
+  std::vector<int> parent;  // coarse box of each fine box
+  int nbuffer = 0;          // amr_nbuffer: flag depth and prolongation zone
+  int fp = -1, fr = -1;     // variable indices of flagprolong (fine), flagrestrict (coarse)
+  // prolongation for nbuffer >= 3, == 1, < 0; restriction for Rflag 3, 2
+  std::vector<ProlongRegion> pro[3];
+  std::vector<RestrictRegion> res[2];
+};
+
 }  // namespace bamhip
 
 struct bamhip_level {
@@ -44,6 +68,7 @@ struct bamhip_level {
   std::vector<double *> v;   // device pointer per variable index
   int *d_flag;               // device scalar used for non-finite detection
   int check_mode = 1;        // 1: every RHS call synchronises and reports
+  bamhip::FmrLink fmr;       // set on a fine level by bamhip_set_rp_flags
 };
 
 namespace bamhip {

@@ -21,12 +21,14 @@ __global__ void fill_kernel(double *__restrict__ p, long n, double v) {
 }
 
 // r = ca*a + cb*b for nv variables (vladd, variables.c:650-719); 2D grid:
-// blockIdx.y selects the variable, the x dimension strides over points
+// blockIdx.y selects the variable, the x dimension strides over points.
+// r may alias a or b (vladd(u, 1, u, c, w) is a common call), so the
+// pointers are not restrict-qualified.
 __global__ void vladd_kernel(Ptrs25 r, Ptrs25 a, Ptrs25 b, double ca, double cb, long n) {
   const int v = blockIdx.y;
-  double *__restrict__ rv = r.p[v];
-  const double *__restrict__ av = a.p[v];
-  const double *__restrict__ bv = b.p[v];
+  double *rv = r.p[v];
+  const double *av = a.p[v];
+  const double *bv = b.p[v];
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     double s = 0.0;
     if (ca != 0.0) s += ca * av[i];

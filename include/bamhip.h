@@ -202,6 +202,51 @@ int bamhip_c2p_batch(const bamhip_c2p_params *par, int method, long n, double *c
 int bamhip_grhd_c2p(bamhip_level *lev, const bamhip_c2p_params *par, int method, const int *cons5,
                     const int *metric6, const int *prim8, int *status, int *iters);
 
+/* ----------------------------------------------------------- FMR coupling */
+/* Replaces setRPflags(lc, lf) (src/main/amr/flag.c:122-262) and links the
+ * fine level lf to its parent level lc (tL::prlocal, tB::pr; box.c:560-652):
+ * fine box b lies in box parent[b] of lc with half its spacing.  Writes
+ * flagprolong (variable fp of lf: depth 1..nbuffer from the fine box
+ * boundary, else 0) and flagrestrict (variable fr of lc: 3, 2, 1 or 0) and
+ * records the transfer regions the varlist calls below use, derived from
+ * these flags as prolong_varlist_boxtobox_N / restrict_varlist_boxtobox_N
+ * derive them (interpolate_boxtobox_N.c:890-935, 1240-1275).  nbuffer is
+ * amr_nbuffer, also the prolongation zone.  Call again after any regrid or
+ * change of the flags.  Fine boxes must have no symmetry or process-ghost
+ * faces. */
+int bamhip_set_rp_flags(bamhip_level *lc, bamhip_level *lf, const int *parent, int nbuffer, int fp, int fr);
+
+/* Replaces restrict_varlist(g, lfine, lcoarse, uf, uc, nbuffer)
+ * (interpolate_boxtobox_N.c:92-180, Lagrange part): order-N Lagrange
+ * restriction (N = order_RP = 2, 4, 6, 8) of the n variables uf of lf into
+ * uc of lc over the bounding box of the coarse points with flagrestrict >= 3
+ * (nbuffer >= 3) or >= 2 (nbuffer == 1).  Bitwise the reference's sweeps. */
+int bamhip_restrict_varlist(bamhip_level *lf, bamhip_level *lc, int n, const int *uf, const int *uc, int nbuffer,
+                            int order);
+
+/* Replaces prolong_varlist(g, lcoarse, lfine, uc, uf, nbuffer) (:184-279):
+ * order-N Lagrange prolongation of uc of lc into uf of lf at the fine points
+ * with 0 < flagprolong <= amr_nbuffer (nbuffer >= 3 or -3), flagprolong == 1
+ * (nbuffer == 1) or everywhere (nbuffer == -1). */
+int bamhip_prolong_varlist(bamhip_level *lc, bamhip_level *lf, int n, const int *uc, const int *uf, int nbuffer,
+                           int order);
+
+/* Replaces restrict_prolong_varlist(g, lcoarse, lfine, uc, uf, nbuffer)
+ * (src/main/amr/inject.c:66-110): restriction, then prolongation. */
+int bamhip_restrict_prolong_varlist(bamhip_level *lc, bamhip_level *lf, int n, const int *uc, const int *uf,
+                                    int nbuffer, int order);
+
+/* Berger-Oliger time interpolation of a coarse level (evolve.c:298-340):
+ * before the level's step, upp = the order-o extrapolation weights times
+ * uc, up, up2.. (o = min(timeinterpolation, iteration + 2), 2..6), then the
+ * history lists shift (up4 <- up3 <- up2 <- up).  Lists that order o does
+ * not use may be NULL. */
+int bamhip_bo_interp_begin(bamhip_level *lev, int timeinterpolation, int iteration, int n, const int *upp,
+                           const int *uc, const int *up, const int *up2, const int *up3, const int *up4);
+/* after the step (evolve.c:367-380): upp += c_o * uc */
+int bamhip_bo_interp_end(bamhip_level *lev, int timeinterpolation, int iteration, int n, const int *upp,
+                         const int *uc);
+
 /* ----------------------------------------------------------------- timers */
 /* accumulated device time per kernel family (BAM's timer_start/timer_stop
  * keys, src/main/bampi/timer.c:60-104), measured with HIP events on the
