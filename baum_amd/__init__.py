@@ -324,6 +324,77 @@ def restrict_prolong_varlist(lc, lf, uc, uf, nbuffer=3, order=4):
           "bamhip_restrict_prolong_varlist")
 
 
+# ---------------------------------------------------------------- hierarchy
+class FmrHierarchyStruct(C.Structure):
+    _fields_ = [("lmax", C.c_int), ("levels", C.POINTER(C.c_void_p)), ("params", C.POINTER(Z4cParams)),
+                ("dt0", C.c_double), ("iteration", C.POINTER(C.c_int)),
+                ("u", IP), ("up", IP), ("upp", IP), ("r", IP), ("v", IP),
+                ("order_RP", C.c_int), ("timeinterpolation", C.c_int), ("pgh", C.c_int)]
+
+
+class Hierarchy:
+    """Berger-Oliger hierarchy of device levels (bamhip_fmr_hierarchy).
+    levels[l] are Level objects, linked with set_rp_flags beforehand;
+    params: one Z4cParams per level."""
+
+    def __init__(self, levels, params, dt0, u, up, upp, r, v, order_RP=4, timeinterpolation=3, pgh=0):
+        L = lib()
+        hp = C.POINTER(FmrHierarchyStruct)
+        _sig(L, "bamhip_advance_levelandsublevels", C.c_int, [hp, C.c_int])
+        _sig(L, "bamhip_fmr_set_check", C.c_int, [hp, C.c_int])
+        _sig(L, "bamhip_fmr_nonfinite", C.c_int, [hp, C.c_int, IP])
+        _sig(L, "bamhip_graph_begin", C.c_int, [])
+        _sig(L, "bamhip_graph_end", C.c_int, [C.POINTER(C.c_void_p)])
+        _sig(L, "bamhip_graph_launch", C.c_int, [C.c_void_p])
+        _sig(L, "bamhip_graph_free", C.c_int, [C.c_void_p])
+        self.levels = list(levels)
+        n = len(self.levels)
+        self._lv = (C.c_void_p * n)(*[lv.h for lv in self.levels])
+        self._par = (Z4cParams * n)(*params)
+        self.iteration = (C.c_int * n)()
+        self._lists = [_iarr(x) for x in (u, up, upp, r, v)]
+        s = FmrHierarchyStruct()
+        s.lmax = n - 1
+        s.levels = C.cast(self._lv, C.POINTER(C.c_void_p))
+        s.params = C.cast(self._par, C.POINTER(Z4cParams))
+        s.dt0 = dt0
+        s.iteration = C.cast(self.iteration, C.POINTER(C.c_int))
+        s.u, s.up, s.upp, s.r, s.v = [C.cast(a, IP) for a in self._lists]
+        s.order_RP, s.timeinterpolation, s.pgh = order_RP, timeinterpolation, pgh
+        self.s = s
+
+    def set_check(self, mode):
+        check(lib().bamhip_fmr_set_check(C.byref(self.s), mode), "bamhip_fmr_set_check")
+
+    def advance(self, l=0):
+        """advance_levelandsublevels(l): one step of level l and its sublevels"""
+        check(lib().bamhip_advance_levelandsublevels(C.byref(self.s), l), "bamhip_advance_levelandsublevels")
+
+    def nonfinite(self, reset=True):
+        f = C.c_int()
+        check(lib().bamhip_fmr_nonfinite(C.byref(self.s), 1 if reset else 0, C.byref(f)), "bamhip_fmr_nonfinite")
+        return f.value
+
+    def capture(self, l=0):
+        """capture one advance(l) into a graph (iteration counters advance as
+        in an eager call); returns the graph handle"""
+        check(lib().bamhip_graph_begin(), "bamhip_graph_begin")
+        try:
+            self.advance(l)
+        finally:
+            g = C.c_void_p()
+            check(lib().bamhip_graph_end(C.byref(g)), "bamhip_graph_end")
+        return g
+
+    @staticmethod
+    def launch(graph):
+        check(lib().bamhip_graph_launch(graph), "bamhip_graph_launch")
+
+    @staticmethod
+    def free_graph(graph):
+        check(lib().bamhip_graph_free(graph), "bamhip_graph_free")
+
+
 # ---------------------------------------------------------------- con2prim
 C2P_COLD, C2P_ATM, C2P_OK, C2P_FAIL, C2P_NR = 0, 1, 2, 3, 4
 C2P_HYBRID, C2P_PLAIN, C2P_BRACKETED, C2P_LEARNED = 0, 1, 2, 3
@@ -402,6 +473,106 @@ def c2p_batch(P, method, cons, metric, prim, status, iters):
                                    C.c_void_p, C.c_void_p]
     check(L.bamhip_c2p_batch(C.byref(P), method, n, cons.ptr, metric.ptr, prim.ptr, status.ptr, iters.ptr),
           "bamhip_c2p_batch")
+
+
+MLP_NPARAM = 4481
+
+
+class Mlp:
+    """the learned pressure guess (bamhip_mlp)"""
+
+    def __init__(self, seed=0):
+        init()
+        L = lib()
+        vp, i, l, f = C.c_void_p, C.c_int, C.c_long, C.c_float
+        FP = C.POINTER(C.c_float)
+        _sig(L, "bamhip_mlp_create", i, [C.POINTER(vp), C.c_ulonglong])
+        _sig(L, "bamhip_mlp_free", i, [vp])
+        _sig(L, "bamhip_mlp_get_params", i, [vp, FP, FP])
+        _sig(L, "bamhip_mlp_set_params", i, [vp, FP, FP])
+        _sig(L, "bamhip_mlp_grad", i, [vp, l, vp, vp, FP, FP])
+        _sig(L, "bamhip_mlp_train", i, [vp, l, vp, vp, i, l, f, C.c_ulonglong, FP])
+        _sig(L, "bamhip_mlp_predict", i, [vp, l, vp, vp])
+        h = vp()
+        check(L.bamhip_mlp_create(C.byref(h), seed), "bamhip_mlp_create")
+        self.h = h
+
+    def free(self):
+        if self.h:
+            lib().bamhip_mlp_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def get(self):
+        p = np.zeros(MLP_NPARAM, np.float32)
+        nrm = np.zeros(8, np.float32)
+        FP = C.POINTER(C.c_float)
+        check(lib().bamhip_mlp_get_params(self.h, p.ctypes.data_as(FP), nrm.ctypes.data_as(FP)),
+              "bamhip_mlp_get_params")
+        return p, nrm
+
+    def set(self, params=None, norm=None):
+        FP = C.POINTER(C.c_float)
+        pa = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        na = None if norm is None else np.ascontiguousarray(norm, dtype=np.float32)
+        check(lib().bamhip_mlp_set_params(self.h, pa.ctypes.data_as(FP) if pa is not None else None,
+                                          na.ctypes.data_as(FP) if na is not None else None),
+              "bamhip_mlp_set_params")
+
+    def grad(self, feat, target):
+        """feat: DeviceArray (n,3) float32, target: DeviceArray (n,) float32"""
+        n = target.shape[0]
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = C.c_float()
+        check(lib().bamhip_mlp_grad(self.h, n, feat.ptr, target.ptr, g.ctypes.data_as(C.POINTER(C.c_float)),
+                                    C.byref(loss)), "bamhip_mlp_grad")
+        return g, loss.value
+
+    def train(self, feat, target, epochs, batch=16384, lr=2e-3, seed=1):
+        n = target.shape[0]
+        el = np.zeros(epochs, np.float32)
+        check(lib().bamhip_mlp_train(self.h, n, feat.ptr, target.ptr, epochs, batch, lr, seed,
+                                     el.ctypes.data_as(C.POINTER(C.c_float))), "bamhip_mlp_train")
+        return el
+
+    def predict(self, feat, y):
+        check(lib().bamhip_mlp_predict(self.h, y.shape[0], feat.ptr, y.ptr), "bamhip_mlp_predict")
+
+
+def mlp_prepare(P, cons, metric, prim, status, feat, target):
+    """raw features (n,3) and targets (n,) of solved points (device arrays)"""
+    n = cons.shape[1]
+    L = lib()
+    vp = C.c_void_p
+    _sig(L, "bamhip_mlp_prepare", C.c_int, [C.POINTER(C2PParams), C.c_long, vp, vp, vp, vp, vp, vp])
+    check(L.bamhip_mlp_prepare(C.byref(P), n, cons.ptr, metric.ptr, prim.ptr, status.ptr, feat.ptr, target.ptr),
+          "bamhip_mlp_prepare")
+
+
+def c2p_learned_batch(P, mlp, cons, metric, prim, status, iters, fallbacks=None, polish_max=8):
+    """learned-guess c2p on device arrays; fallbacks: DeviceArray (1,) int32 accumulating the count"""
+    n = cons.shape[1]
+    L = lib()
+    vp = C.c_void_p
+    _sig(L, "bamhip_c2p_learned_batch", C.c_int,
+         [C.POINTER(C2PParams), vp, C.c_int, C.c_long, vp, vp, vp, vp, vp, vp])
+    check(L.bamhip_c2p_learned_batch(C.byref(P), mlp.h, polish_max, n, cons.ptr, metric.ptr, prim.ptr, status.ptr,
+                                     iters.ptr, fallbacks.ptr if fallbacks is not None else None),
+          "bamhip_c2p_learned_batch")
+
+
+def p2c_batch(P, prim5, metric, cons):
+    """device-array prim -> cons: prim5 (5,n), metric (6,n), cons (5,n) out"""
+    n = cons.shape[1]
+    L = lib()
+    L.bamhip_p2c_batch.restype = C.c_int
+    L.bamhip_p2c_batch.argtypes = [C.POINTER(C2PParams), C.c_long, C.c_void_p, C.c_void_p, C.c_void_p]
+    check(L.bamhip_p2c_batch(C.byref(P), n, prim5.ptr, metric.ptr, cons.ptr), "bamhip_p2c_batch")
 
 
 # ------------------------------------------------------------------ timers

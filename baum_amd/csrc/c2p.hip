@@ -17,6 +17,7 @@
 #include <cmath>
 
 #include "bamhip_internal.h"
+#include "mlp_device.h"
 
 #pragma clang fp contract(off)
 
@@ -337,9 +338,9 @@ __global__ void c2p_plain_kernel(C2PArrays A, C2PK P, long n) {
 // Newton step leaving it is replaced by bisection, and the iteration stops
 // when |f| is within its rounding floor (sqrt(temp^2 - S^2) has a relative
 // error ~ eps_mach W^2), or the relative update / bracket is below
-// errormax, or after countmax iterations (NR).  guess <= 0: start at plo.
+// errormax, or after maxit iterations (NR).  guess <= 0: start at plo.
 __device__ int c2p_bracketed_pt(const C2PK &P, double conD, double conS2, double conT, double guess, double *rho,
-                                double *eps, double *pres, int *iters) {
+                                double *eps, double *pres, int *iters, int maxit) {
   double plo = sqrt(conS2) - conD - conT;
   if (plo < 0.) plo = 0.;
   double phi = (P.gamma - 1.) * (conT + conD);
@@ -348,7 +349,7 @@ __device__ int c2p_bracketed_pt(const C2PK &P, double conD, double conS2, double
   const double em = 2.220446049250313e-16;
   int count;
   *iters = 0;
-  for (count = 0; count < P.countmax; count++) {
+  for (count = 0; count < maxit; count++) {
     const double temp = conT + p + conD;
     const double arg = temp * temp - conS2;
     const double sq = sqrt(arg);
@@ -379,7 +380,7 @@ __device__ int c2p_bracketed_pt(const C2PK &P, double conD, double conS2, double
   *rho = conD / W;
   *eps = (sq - W * p) / conD - 1.;
   *pres = p;
-  if (count >= P.countmax) return C2P_NR;
+  if (count >= maxit) return C2P_NR;
   if (!isfinite(p) || !isfinite(*rho) || !isfinite(*eps)) return C2P_FAIL;
   return C2P_OK;
 }
@@ -427,10 +428,105 @@ __global__ void c2p_bracketed_kernel(C2PArrays A, C2PK P, long n) {
     const Undens u = undensitise(A, q);
     double rho, eps, p;
     int it;
-    const int st = c2p_bracketed_pt(P, u.D, u.S2, u.T, A.prim[5][q], &rho, &eps, &p, &it);
+    const int st = c2p_bracketed_pt(P, u.D, u.S2, u.T, A.prim[5][q], &rho, &eps, &p, &it, P.countmax);
     store_prims(A, q, u, rho, eps, p);
     A.status[q] = st;
     A.iters[q] = it;
+  }
+}
+
+// Learned guess + bracketed polish: each wavefront evaluates the network
+// for its 64 points on the matrix cores (mlp_device.h), then every lane
+// polishes its point with the bracketed Newton solver started at the
+// predicted pressure, limited to polish_max iterations.  A point whose
+// features are not finite or whose polish does not converge falls back to
+// the cold-started bracketed solve (counted in *fallbacks).  The loop trip
+// count is uniform over the block, so every wave reaches the barriers.
+__global__ void __launch_bounds__(64) c2p_learned_kernel(C2PArrays A, C2PK P, MlpView M, long n, int polish_max,
+                                                         int *fallbacks) {
+  // one wavefront per block: the barriers only order this wave's LDS traffic
+  __shared__ __attribute__((aligned(16))) __bf16 h1[64 * kHStr];
+  const int l = threadIdx.x;
+  int nfb = 0;
+  for (long base = (long)blockIdx.x * 64; base < n; base += (long)gridDim.x * 64) {
+    const long q = base + l;
+    const bool act = q < n;
+    Undens u = {};
+    double th = 0.;
+    float x[3] = {0.f, 0.f, 0.f};
+    bool valid = false;
+    if (act) {
+      u = undensitise(A, q);
+      th = mlp_thermal(u.D, u.S2, u.T);
+      valid = mlp_features(M, u.D, u.S2, u.T, th, x);
+      if (!valid) x[0] = x[1] = x[2] = 0.f;
+    }
+    mlp_layer1(M, x, h1 + l * kHStr);
+    __syncthreads();
+    f32x16 c[2][2];
+    const float yn = mlp_layer23(M, h1, c);
+    __syncthreads();
+    if (act) {
+      double rho = 0., eps = 0., p = 0.;
+      int it = 0, it2 = 0, st = C2P_NR;
+      if (valid) {
+        const double guess = (P.gamma - 1.) * th * (double)__expf(M.ys + M.yk * yn);
+        st = c2p_bracketed_pt(P, u.D, u.S2, u.T, guess, &rho, &eps, &p, &it, polish_max);
+      }
+      if (st != C2P_OK) {
+        nfb++;
+        st = c2p_bracketed_pt(P, u.D, u.S2, u.T, 0., &rho, &eps, &p, &it2, P.countmax);
+      }
+      store_prims(A, q, u, rho, eps, p);
+      A.status[q] = st;
+      A.iters[q] = it + it2;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) nfb += __shfl_xor(nfb, o);
+  if (fallbacks && l == 0 && nfb) atomicAdd(fallbacks, nfb);
+}
+
+// training data of the learned guess from solved points: raw features
+// (log D, |S|/(tau+D), log(Delta/(tau+D))) and target
+// log(p / ((Gamma-1) Delta)); the target is NaN where the solve did not
+// succeed or the state has no thermal content
+__global__ void mlp_prepare_kernel(C2PArrays A, C2PK P, long n, float *__restrict__ feat, float *__restrict__ target) {
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const Undens u = undensitise(A, q);
+    const double th = mlp_thermal(u.D, u.S2, u.T);
+    float r[3];
+    const bool fok = mlp_raw_features(u.D, u.S2, u.T, th, r);
+    for (int j = 0; j < 3; j++) feat[3 * q + j] = r[j];
+    const bool ok = A.status[q] == C2P_OK && fok;
+    target[q] = ok ? (float)log(A.prim[5][q] / ((P.gamma - 1.) * th)) : __builtin_nanf("");
+  }
+}
+
+// primitives -> conserved variables: grhd_compute_v2_pt, EOS p(rho, eps),
+// invg and grhd_compute_q_pt (grhd_p2c.c:11-25,68-86); prim5 = rho, eps, v^i
+__global__ void p2c_kernel(C2PK P, long n, const double *__restrict__ prim5, const double *__restrict__ metric,
+                           double *__restrict__ cons) {
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const double rho = prim5[q], eps = prim5[n + q];
+    const double vx = prim5[2 * n + q], vy = prim5[3 * n + q], vz = prim5[4 * n + q];
+    const double gxx = metric[q], gxy = metric[n + q], gxz = metric[2 * n + q];
+    const double gyy = metric[3 * n + q], gyz = metric[4 * n + q], gzz = metric[5 * n + q];
+    double p, chi, kappa;
+    eos_re(P, rho, eps, &p, &chi, &kappa);
+    const double vlx = gxx * vx + gxy * vy + gxz * vz;
+    const double vly = gxy * vx + gyy * vy + gyz * vz;
+    const double vlz = gxz * vx + gyz * vy + gzz * vz;
+    const double v2 = vlx * vx + vly * vy + vlz * vz;
+    const double W = 1.0 / sqrt(1.0 - v2);
+    double i11, i12, i13, i22, i23, i33;
+    const double detg = invg(gxx, gxy, gxz, gyy, gyz, gzz, &i11, &i12, &i13, &i22, &i23, &i33);
+    const double sq = sqrt(detg);
+    const double W2hrho = W * W * (rho + rho * eps + p);
+    cons[q] = sq * W * rho;
+    cons[n + q] = sq * (W2hrho - p - W * rho);
+    cons[2 * n + q] = sq * W2hrho * vlx;
+    cons[3 * n + q] = sq * W2hrho * vly;
+    cons[4 * n + q] = sq * W2hrho * vlz;
   }
 }
 
@@ -485,6 +581,84 @@ extern "C" int bamhip_c2p_batch(const bamhip_c2p_params *par, int method, long n
   A.status = status;
   A.iters = iters;
   return c2p_launch(par, method, n, A);
+}
+
+static void c2p_flat(C2PArrays &A, long n, double *cons, const double *metric, double *prim, int *status,
+                     int *iters) {
+  for (int k = 0; k < 5; k++) A.cons[k] = cons + k * n;
+  for (int k = 0; k < 6; k++) A.g[k] = metric + k * n;
+  for (int k = 0; k < 8; k++) A.prim[k] = prim + k * n;
+  A.status = status;
+  A.iters = iters;
+}
+
+static int learned_launch(const bamhip_c2p_params *par, const bamhip_mlp *m, int polish_max, long n, C2PArrays &A,
+                          int *fallbacks) {
+  if (!par || !m || n < 0 || polish_max < 1) return set_error(BAMHIP_EINVAL, "bamhip_c2p_learned: bad arguments");
+  if (par->eos != 0) return set_error(BAMHIP_EINVAL, "bamhip_c2p_learned: the learned guess is trained for the ideal gas");
+  if (par->countmax < 1) return set_error(BAMHIP_EINVAL, "bamhip_c2p_learned: countmax < 1");
+  if (n == 0) return BAMHIP_OK;
+  long blocks = (n + 63) / 64;
+  if (blocks > 32768) blocks = 32768;
+  TimerScope t("c2p");
+  c2p_learned_kernel<<<(unsigned)blocks, 64, 0, stream()>>>(A, make_c2pk(par), mlp_view(m), n, polish_max,
+                                                            fallbacks);
+  BH_HIP(hipGetLastError());
+  return BAMHIP_OK;
+}
+
+extern "C" int bamhip_c2p_learned_batch(const bamhip_c2p_params *par, const bamhip_mlp *m, int polish_max, long n,
+                                        double *cons, const double *metric, double *prim, int *status, int *iters,
+                                        int *fallbacks) {
+  if (!cons || !metric || !prim || !status || !iters)
+    return set_error(BAMHIP_EINVAL, "bamhip_c2p_learned_batch: NULL array");
+  C2PArrays A;
+  c2p_flat(A, n, cons, metric, prim, status, iters);
+  return learned_launch(par, m, polish_max, n, A, fallbacks);
+}
+
+extern "C" int bamhip_grhd_c2p_learned(bamhip_level *lev, const bamhip_c2p_params *par, const bamhip_mlp *m,
+                                       int polish_max, const int *cons5, const int *metric6, const int *prim8,
+                                       int *status, int *iters, int *fallbacks) {
+  if (!lev) return set_error(BAMHIP_EINVAL, "bamhip_grhd_c2p_learned: NULL level");
+  if (!status || !iters) return set_error(BAMHIP_EINVAL, "bamhip_grhd_c2p_learned: status/iters buffers are required");
+  C2PArrays A;
+  double *tmp[8];
+  int e;
+  if ((e = gather_ptrs(lev, cons5, 5, tmp, "bamhip_grhd_c2p_learned(cons)"))) return e;
+  for (int k = 0; k < 5; k++) A.cons[k] = tmp[k];
+  if ((e = gather_ptrs(lev, metric6, 6, tmp, "bamhip_grhd_c2p_learned(metric)"))) return e;
+  for (int k = 0; k < 6; k++) A.g[k] = tmp[k];
+  if ((e = gather_ptrs(lev, prim8, 8, tmp, "bamhip_grhd_c2p_learned(prim)"))) return e;
+  for (int k = 0; k < 8; k++) A.prim[k] = tmp[k];
+  A.status = status;
+  A.iters = iters;
+  return learned_launch(par, m, polish_max, lev->npoints, A, fallbacks);
+}
+
+extern "C" int bamhip_mlp_prepare(const bamhip_c2p_params *par, long n, const double *cons, const double *metric,
+                                  const double *prim, const int *status, float *feat, float *target) {
+  if (!par || n < 0 || !cons || !metric || !prim || !status || !feat || !target)
+    return set_error(BAMHIP_EINVAL, "bamhip_mlp_prepare: bad arguments");
+  if (n == 0) return BAMHIP_OK;
+  C2PArrays A;
+  c2p_flat(A, n, const_cast<double *>(cons), metric, const_cast<double *>(prim), const_cast<int *>(status), nullptr);
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  mlp_prepare_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(A, make_c2pk(par), n, feat, target);
+  BH_HIP(hipGetLastError());
+  return BAMHIP_OK;
+}
+
+extern "C" int bamhip_p2c_batch(const bamhip_c2p_params *par, long n, const double *prim5, const double *metric,
+                                double *cons) {
+  if (!par || n < 0 || !prim5 || !metric || !cons) return set_error(BAMHIP_EINVAL, "bamhip_p2c_batch: bad arguments");
+  if (n == 0) return BAMHIP_OK;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  p2c_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(make_c2pk(par), n, prim5, metric, cons);
+  BH_HIP(hipGetLastError());
+  return BAMHIP_OK;
 }
 
 extern "C" int bamhip_grhd_c2p(bamhip_level *lev, const bamhip_c2p_params *par, int method, const int *cons5,

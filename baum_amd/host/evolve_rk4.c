@@ -10,32 +10,42 @@
  */
 #include "../../include/bamhip.h"
 
-int bamhip_z4c_rk4_step(bamhip_level *lev, const bamhip_z4c_params *par, double dt, const int *uc, const int *up,
-                        const int *r, const int *v, int pgh, int algcon)
+int bamhip_z4c_rk4_stages(bamhip_level *lev, const bamhip_z4c_params *par, double dt, const int *uc, const int *up,
+                          const int *r, const int *v, int pgh)
 {
   /* stage weights of rk4 (rk.c:98-103): b = 1/6 1/3 1/3 1/6, c = 1/2 1/2 1 */
   const double wc[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
   const double wv[4] = {dt / 2, dt / 2, dt, 0.0};
-  int e, flag = 0, s;
-  int gi[6], ai[6];
-
-  if ((e = bamhip_level_set_check(lev, 0))) return e;
-  if ((e = bamhip_level_nonfinite(lev, 1, 0))) goto out;
+  int e, s;
   /* u_p = u_c */
-  if ((e = bamhip_vladd(lev, BAMHIP_Z4C_NVARS, up, 1.0, uc, 0.0, 0))) goto out;
+  if ((e = bamhip_vladd(lev, BAMHIP_Z4C_NVARS, up, 1.0, uc, 0.0, 0))) return e;
   for (s = 0; s < 4; s++) {
     const int *src = (s == 0) ? uc : v;
-    if ((e = bamhip_z4c_rhs(lev, par, r, 0, 0.0, src))) goto out;
-    if ((e = bamhip_z4c_set_boundary(lev, par, r, 0, 0.0, src, pgh))) goto out;
-    if ((e = bamhip_rk4_stage_update(lev, BAMHIP_Z4C_NVARS, uc, up, r, v, wc[s], wv[s]))) goto out;
+    if ((e = bamhip_z4c_rhs(lev, par, r, 0, 0.0, src))) return e;
+    if ((e = bamhip_z4c_set_boundary(lev, par, r, 0, 0.0, src, pgh))) return e;
+    if ((e = bamhip_rk4_stage_update(lev, BAMHIP_Z4C_NVARS, uc, up, r, v, wc[s], wv[s]))) return e;
   }
-  if (algcon) {
-    for (s = 0; s < 6; s++) {
-      gi[s] = uc[s];
-      ai[s] = uc[7 + s];
-    }
-    if ((e = bamhip_z4c_algcon(lev, gi, ai, par->subtractA, par->normalizedetg))) goto out;
+  return BAMHIP_OK;
+}
+
+int bamhip_z4c_algcon_list(bamhip_level *lev, const bamhip_z4c_params *par, const int *uc)
+{
+  int gi[6], ai[6], s;
+  for (s = 0; s < 6; s++) {
+    gi[s] = uc[s];
+    ai[s] = uc[7 + s];
   }
+  return bamhip_z4c_algcon(lev, gi, ai, par->subtractA, par->normalizedetg);
+}
+
+int bamhip_z4c_rk4_step(bamhip_level *lev, const bamhip_z4c_params *par, double dt, const int *uc, const int *up,
+                        const int *r, const int *v, int pgh, int algcon)
+{
+  int e, flag = 0;
+  if ((e = bamhip_level_set_check(lev, 0))) return e;
+  if ((e = bamhip_level_nonfinite(lev, 1, 0))) goto out;
+  if ((e = bamhip_z4c_rk4_stages(lev, par, dt, uc, up, r, v, pgh))) goto out;
+  if (algcon && (e = bamhip_z4c_algcon_list(lev, par, uc))) goto out;
   if ((e = bamhip_level_nonfinite(lev, 0, &flag))) goto out;
   if (flag) e = BAMHIP_ENONFINITE;
 out:

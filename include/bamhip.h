@@ -194,6 +194,13 @@ enum { BAMHIP_C2P_COLD = 0, BAMHIP_C2P_ATM = 1, BAMHIP_C2P_OK = 2, BAMHIP_C2P_FA
 int bamhip_c2p_batch(const bamhip_c2p_params *par, int method, long n, double *cons,
                      const double *metric, double *prim, int *status, int *iters);
 
+/* primitives -> conserved variables on flat device arrays of n points:
+ * grhd_compute_v2_pt + EOS + grhd_compute_q_pt (src/projects/grhd/
+ * grhd_p2c.c:11-25,68-86).  prim5[5n] = rho, eps, v^x, v^y, v^z
+ * (contravariant), metric[6n], cons[5n] = densitised D, tau, S_i. */
+int bamhip_p2c_batch(const bamhip_c2p_params *par, long n, const double *prim5, const double *metric,
+                     double *cons);
+
 /* Replaces MATTER.c2p(ucur, primVars) (src/projects/matter/bam_matter.h:49,
  * called at matter.c:317,411) on a level: cons5 / metric6 / prim8 are the
  * variable indices of the conserved variables, of adm_gxx.. and of the
@@ -201,6 +208,50 @@ int bamhip_c2p_batch(const bamhip_c2p_params *par, int method, long n, double *c
  * status / iters: device arrays of npoints ints. */
 int bamhip_grhd_c2p(bamhip_level *lev, const bamhip_c2p_params *par, int method, const int *cons5,
                     const int *metric6, const int *prim8, int *status, int *iters);
+
+/* ----------------------------------------------- learned pressure guess */
+/* A 3 -> 64 -> 64 -> 1 ReLU network predicting log(p / ((Gamma-1) Delta)),
+ * Delta = tau + D - sqrt(S^2 + D^2) the state's thermal content (zero for a
+ * cold flow), from log D, |S|/(tau+D) and log(Delta/(tau+D)), all formed in
+ * FP64 from the undensitised state (inputs and output normalised by
+ * the 8 floats `norm`: input shift[3], input scale[3], output shift, output
+ * scale).  bf16 weights with FP32 masters, FP32 accumulation, trained on
+ * the device with Adam.  No reference counterpart: it replaces only the
+ * initial pressure guess of the bracketed solver, which polishes it in
+ * FP64 and remains the fallback. */
+typedef struct bamhip_mlp bamhip_mlp;
+#define BAMHIP_MLP_NPARAM 4481  /* W1[64][3] b1[64] W2[64][64] b2[64] W3[64] b3 */
+
+int bamhip_mlp_create(bamhip_mlp **m, unsigned long long seed);
+int bamhip_mlp_free(bamhip_mlp *m);
+/* host copies of the FP32 master parameters and of norm[8] (either may be NULL) */
+int bamhip_mlp_get_params(const bamhip_mlp *m, float *params, float *norm);
+/* set parameters (resets the optimiser) and/or norm */
+int bamhip_mlp_set_params(bamhip_mlp *m, const float *params, const float *norm);
+/* training data from solved points (device arrays of n points, layouts of
+ * bamhip_c2p_batch): feat[3n] raw features, target[n] (NaN unless status OK) */
+int bamhip_mlp_prepare(const bamhip_c2p_params *par, long n, const double *cons, const double *metric,
+                       const double *prim, const int *status, float *feat, float *target);
+/* gradient of the mean squared error over all n samples (host grad[NPARAM],
+ * mean loss), without changing the model */
+int bamhip_mlp_grad(bamhip_mlp *m, long n, const float *feat, const float *target, float *grad, float *loss);
+/* Adam (cosine-decayed learning rate) over `epochs` passes in minibatches;
+ * epoch_loss[epochs] (may be NULL) receives the last minibatch's loss */
+int bamhip_mlp_train(bamhip_mlp *m, long n, const float *feat, const float *target, int epochs, long batch,
+                     float lr, unsigned long long seed, float *epoch_loss);
+/* network output log(p / p_hi) for raw features (device arrays) */
+int bamhip_mlp_predict(const bamhip_mlp *m, long n, const float *feat, float *y);
+/* con2prim with the learned guess, polished by the bracketed solver in at
+ * most polish_max iterations; failed polishes fall back to the cold-started
+ * bracketed solve and are counted in *fallbacks (device int, may be NULL).
+ * Arrays as bamhip_c2p_batch; ideal-gas EOS only. */
+int bamhip_c2p_learned_batch(const bamhip_c2p_params *par, const bamhip_mlp *m, int polish_max, long n,
+                             double *cons, const double *metric, double *prim, int *status, int *iters,
+                             int *fallbacks);
+/* the same on the variables of a level (as bamhip_grhd_c2p) */
+int bamhip_grhd_c2p_learned(bamhip_level *lev, const bamhip_c2p_params *par, const bamhip_mlp *m, int polish_max,
+                            const int *cons5, const int *metric6, const int *prim8, int *status, int *iters,
+                            int *fallbacks);
 
 /* ----------------------------------------------------------- FMR coupling */
 /* Replaces setRPflags(lc, lf) (src/main/amr/flag.c:122-262) and links the
@@ -246,6 +297,54 @@ int bamhip_bo_interp_begin(bamhip_level *lev, int timeinterpolation, int iterati
 /* after the step (evolve.c:367-380): upp += c_o * uc */
 int bamhip_bo_interp_end(bamhip_level *lev, int timeinterpolation, int iteration, int n, const int *upp,
                          const int *uc);
+
+/* ------------------------------------------------ FMR hierarchy evolution */
+/* The levels of a Berger-Oliger hierarchy (tG with lmin = 0): levels[l] for
+ * l = 0..lmax, each fine level linked to its parent by bamhip_set_rp_flags.
+ * Level l steps with dt0 / 2^l; iteration[l] is tL::iteration (updated by
+ * the calls).  u, up, upp, r, v: the 25 Z4c indices of the state, the saved
+ * state, the parent's time-interpolated state and the RK4 registers, the
+ * same on every level.  params[l]: the RHS parameters of level l (the
+ * dissipation factor may differ per level).  pgh: periodic ghosts of level 0
+ * (0: radiative boundary). */
+typedef struct {
+  int lmax;
+  bamhip_level *const *levels;
+  const bamhip_z4c_params *params;
+  double dt0;
+  int *iteration;
+  const int *u, *up, *upp, *r, *v;
+  int order_RP;            /* prolongation / restriction order (4) */
+  int timeinterpolation;   /* order_timeinterpolation: 2 or 3 */
+  int pgh;
+} bamhip_fmr_hierarchy;
+
+/* Replaces advance_levelandsublevels(g, l) (src/main/main/main.c:397-453)
+ * for the Z4c list: one step of level l (time interpolation of u_pp around
+ * an RK4 step, evolve.c:298-380, then algcon), two steps of every finer
+ * level, then restrict_prolong_evolve with the parent (inject.c:116-132).
+ * Device work only: the RHS non-finite check is deferred (levels must be in
+ * check mode 0, see bamhip_fmr_set_check), so the call can be captured. */
+int bamhip_advance_levelandsublevels(const bamhip_fmr_hierarchy *H, int l);
+int bamhip_fmr_set_check(const bamhip_fmr_hierarchy *H, int mode);
+/* OR of the levels' non-finite flags (synchronises); reset clears them */
+int bamhip_fmr_nonfinite(const bamhip_fmr_hierarchy *H, int reset, int *flag);
+
+/* RK4 stages without algcon and without the non-finite check, and algcon on
+ * the 12 fields of a Z4c list (building blocks of the drivers above) */
+int bamhip_z4c_rk4_stages(bamhip_level *lev, const bamhip_z4c_params *par, double dt, const int *uc, const int *up,
+                          const int *r, const int *v, int pgh);
+int bamhip_z4c_algcon_list(bamhip_level *lev, const bamhip_z4c_params *par, const int *uc);
+
+/* --------------------------------------------------------------- graphs */
+/* Capture the device work issued on the library stream between begin and
+ * end into an executable graph (hipGraph), replayed by launch.  Timers are
+ * paused during capture; work that synchronises with the host (check mode
+ * 1, put/get) fails while capturing. */
+int bamhip_graph_begin(void);
+int bamhip_graph_end(void **graph);
+int bamhip_graph_launch(void *graph);
+int bamhip_graph_free(void *graph);
 
 /* ----------------------------------------------------------------- timers */
 /* accumulated device time per kernel family (BAM's timer_start/timer_stop

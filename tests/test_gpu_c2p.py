@@ -104,3 +104,18 @@ def test_c2p_polytrope_hybrid():
     assert np.mean(gs == cs) > 0.999
     same = gs == cs
     np.testing.assert_allclose(gp[:, same], cp[:, same], rtol=1e-11, atol=1e-300)
+
+
+def test_p2c_matches_oracle_exactly():
+    """grhd_compute_q_pt on the device (the generator of the c2p benchmark
+    inputs) against the oracle's restatement"""
+    n = 50000
+    prim5, g, _ = sample_prims(n, seed=21)
+    P = O.c2p_params(errormax=1e-10)
+    Pg = B.c2p_params(errormax=1e-10)
+    dp, dg, dc = B.DeviceArray.from_host(prim5), B.DeviceArray.from_host(g), B.DeviceArray((5, n))
+    B.p2c_batch(Pg, dp, dg, dc)
+    got = dc.get()
+    for a in (dp, dg, dc):
+        a.free()
+    np.testing.assert_array_equal(got, O.p2c(P, prim5, g))
