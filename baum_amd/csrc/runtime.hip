@@ -81,7 +81,9 @@ static void resolve_pending() {
   g_pending.clear();
 }
 
-TimerScope::TimerScope(const char *name) : name_(name), a_(nullptr), b_(nullptr), on_(g_timer_on) {
+static bool g_capturing = false;   // the library stream is being captured into a graph
+
+TimerScope::TimerScope(const char *name) : name_(name), a_(nullptr), b_(nullptr), on_(g_timer_on && !g_capturing) {
   if (!on_) return;
   std::lock_guard<std::mutex> lk(g_timer_mu);
   a_ = pool_get();
@@ -147,6 +149,38 @@ int bamhip_malloc(void **ptr, size_t bytes) {
 
 int bamhip_free(void *ptr) {
   if (ptr) BH_HIP(hipFree(ptr));
+  return BAMHIP_OK;
+}
+
+int bamhip_graph_begin(void) {
+  if (!g_stream) return set_error(BAMHIP_EINVAL, "bamhip_graph_begin: call bamhip_init first");
+  if (g_capturing) return set_error(BAMHIP_EINVAL, "bamhip_graph_begin: already capturing");
+  BH_HIP(hipStreamBeginCapture(g_stream, hipStreamCaptureModeRelaxed));
+  g_capturing = true;
+  return BAMHIP_OK;
+}
+
+int bamhip_graph_end(void **graph) {
+  if (!g_capturing || !graph) return set_error(BAMHIP_EINVAL, "bamhip_graph_end: not capturing");
+  g_capturing = false;
+  hipGraph_t g = nullptr;
+  BH_HIP(hipStreamEndCapture(g_stream, &g));
+  hipGraphExec_t x = nullptr;
+  const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  BH_HIP(e);
+  *graph = (void *)x;
+  return BAMHIP_OK;
+}
+
+int bamhip_graph_launch(void *graph) {
+  if (!graph) return set_error(BAMHIP_EINVAL, "bamhip_graph_launch: NULL graph");
+  BH_HIP(hipGraphLaunch((hipGraphExec_t)graph, g_stream));
+  return BAMHIP_OK;
+}
+
+int bamhip_graph_free(void *graph) {
+  if (graph) BH_HIP(hipGraphExecDestroy((hipGraphExec_t)graph));
   return BAMHIP_OK;
 }
 

@@ -42,3 +42,16 @@ def centred_window(xi, n):
     the Lagrange evaluation point inside that window"""
     s = int(np.floor(xi - (n - 1) / 2 + 0.5))
     return s, xi - s
+
+
+def small_hierarchy(mk0, mkf):
+    """three levels around a Brill-Lindquist binary at x = -3, +3: a 32^3
+    coarse box (H = 0.5, made by mk0), two uneven boxes on level 1 and one
+    box inside each of them on level 2 (made by mkf).  Returns (boxes per
+    level, parent index of every box per level)."""
+    L0 = [mk0(32, 32, 32, -7.75, -7.75, -7.75, 0.5)]
+    a1 = fine_box(mkf, L0[0], (4, 12, 12), (16, 16, 16))
+    b1 = fine_box(mkf, L0[0], (18, 12, 11), (20, 16, 18))
+    a2 = fine_box(mkf, a1, (4, 4, 4), (16, 16, 16))
+    b2 = fine_box(mkf, b1, (6, 5, 5), (16, 12, 14))
+    return [L0, [a1, b1], [a2, b2]], [[0], [0, 0], [0, 1]]

@@ -1,0 +1,143 @@
+"""Berger-Oliger evolution of a three-level FMR hierarchy on the device
+(bamhip_advance_levelandsublevels) against the oracle composed step by step
+(RK4 per box, time interpolation, algcon, restriction and prolongation in
+the reference's order), and graph replay of a coarse step against eager
+execution."""
+import numpy as np
+import pytest
+
+import baum_amd as B
+import oracle as O
+from fmr_geometry import small_hierarchy
+from spacetimes import grid, puncture_z4c
+
+pytestmark = pytest.mark.gpu
+
+U, UP, UPP, R, V = (list(range(25 * k, 25 * k + 25)) for k in range(5))
+FP, FR = 125, 126
+NVAR = 127
+NB, ORDER, TI = 6, 4, 3
+BINARY = ((0.5, (-3.0, 0.0, 0.0)), (0.5, (3.0, 0.0, 0.0)))
+
+
+def _initial(boxes):
+    return [np.concatenate([puncture_z4c(grid(b), BINARY).reshape(25, -1) for b in lev], axis=1) for lev in boxes]
+
+
+def _gpu_hierarchy(dt0):
+    mk0 = lambda *a: B.make_box(*a, bflag=(B.PHYBOUND,) * 6)
+    mkf = lambda *a: B.make_box(*a, bflag=(B.REFBOUND,) * 6)
+    boxes, parents = small_hierarchy(mk0, mkf)
+    levels = []
+    for l, bl in enumerate(boxes):
+        lv = B.Level(bl, NVAR, l=l)
+        lv.enable(range(NVAR))
+        for v in range(NVAR):
+            lv.fill(v, 0.0)
+        levels.append(lv)
+    for lv, u in zip(levels, _initial(boxes)):
+        lv.put_state(U, u)
+    for l in range(1, len(levels)):
+        B.set_rp_flags(levels[l - 1], levels[l], parents[l], NB, FP, FR)
+    par = B.z4c_params()
+    H = B.Hierarchy(levels, [par] * len(levels), dt0, U, UP, UPP, R, V, order_RP=ORDER, timeinterpolation=TI)
+    H.set_check(0)
+    return H, levels
+
+
+class OracleHierarchy:
+    """the same recursion with the CPU oracle (main.c:397-453, evolve.c:298-380,
+    inject.c:116-132)"""
+
+    def __init__(self, dt0):
+        self.boxes, self.parents = small_hierarchy(O.make_box, O.make_box)
+        self.u = _initial(self.boxes)
+        self.upp = [np.zeros_like(u) for u in self.u]
+        self.work = [[np.zeros((75, b.o, b.n, b.m)) for b in lev] for lev in self.boxes]
+        self.it = [0] * len(self.boxes)
+        self.dt0 = dt0
+        self.par = O.z4c_params()
+        self.flags = [None] + [O.rp_flags(self.boxes[l], self.boxes[l - 1], self.parents[l], NB)
+                               for l in range(1, len(self.boxes))]
+
+    def _level_step(self, l):
+        lmax = len(self.boxes) - 1
+        dt = self.dt0 / 2 ** l
+        o = min(TI, self.it[l] + 2)
+        off = 0
+        for b, box in enumerate(self.boxes[l]):
+            npb = box.m * box.n * box.o
+            sl = slice(off, off + npb)
+            ub = np.ascontiguousarray(self.u[l][:, sl].reshape(25, box.o, box.n, box.m))
+            upb = self.work[l][b][0:25]
+            if l < lmax:
+                upp = 0.5 * ub if o == 2 else 0.75 * ub + (-0.125) * upb
+            assert O.z4c_rk4_step(box, self.par, dt, ub, bc=1 if l == 0 else 0, pgh=0, algcon=False,
+                                  work=self.work[l][b]) == 0
+            if l < lmax:
+                upp = upp + (0.5 if o == 2 else 0.375) * ub
+                self.upp[l][:, sl] = upp.reshape(25, -1)
+            O.z4c_algcon(box, ub)
+            self.u[l][:, sl] = ub.reshape(25, -1)
+            off += npb
+        self.it[l] += 1
+
+    def advance(self, l=0):
+        self._level_step(l)
+        if l < len(self.boxes) - 1:
+            self.advance(l + 1)
+            self.advance(l + 1)
+        if l > 0:
+            fp, fr = self.flags[l]
+            aligned = self.it[l] == 2 * self.it[l - 1]
+            cv = self.u[l - 1] if aligned else self.upp[l - 1]
+            args = (self.boxes[l], self.boxes[l - 1], self.parents[l])
+            assert O.restrict_level(*args, fr, 3, ORDER, self.u[l], cv) == 0
+            assert O.prolong_level(*args, fp, 3, NB, ORDER, cv, self.u[l]) == 0
+
+
+def _rel_err(a, b):
+    scale = np.maximum(np.abs(b).max(axis=1), 1e-300)
+    return np.abs(a - b).max(axis=1) / scale
+
+
+def test_hierarchy_step_matches_oracle():
+    dt0 = 0.125
+    H, levels = _gpu_hierarchy(dt0)
+    ref = OracleHierarchy(dt0)
+    u0 = [u.copy() for u in ref.u]
+    for step in range(2):
+        H.advance(0)
+        ref.advance(0)
+        assert H.nonfinite() == 0
+        assert list(H.iteration) == ref.it == [step + 1, 2 * (step + 1), 4 * (step + 1)]
+        for l, lv in enumerate(levels):
+            got = lv.get_state(U)
+            err = _rel_err(got - u0[l], ref.u[l] - u0[l])
+            assert np.all(np.isfinite(got))
+            assert err.max() <= 1e-10, (step, l, dict(zip(B.VAR_NAMES, err)))
+    # the fine levels actually evolved and the buffers were refilled
+    assert np.abs(levels[2].get(U[18]) - u0[2][18]).max() > 1e-4
+
+
+def test_graph_replay_matches_eager():
+    dt0 = 0.125
+    HA, LA = _gpu_hierarchy(dt0)
+    HB, LB = _gpu_hierarchy(dt0)
+    HA.advance(0)            # first step: time interpolation of reduced order
+    HB.advance(0)
+    g = HA.capture(0)        # records the second coarse step, executes nothing
+    try:
+        for step in range(3):
+            HA.launch(g)
+            HB.advance(0)
+            # eager work and host synchronisation between replays
+            l2a = LA[0].z4c_ham_l2(U)
+            l2b = LB[0].z4c_ham_l2(U)
+            assert l2a == l2b
+            for la, lb in zip(LA, LB):
+                np.testing.assert_array_equal(la.get_state(U), lb.get_state(U))
+                np.testing.assert_array_equal(la.get_state(UPP), lb.get_state(UPP))
+    finally:
+        B.Hierarchy.free_graph(g)
+    assert HA.nonfinite() == 0
