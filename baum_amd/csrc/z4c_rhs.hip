@@ -87,6 +87,15 @@ __device__ __forceinline__ constexpr int mixid(int a, int b) {  // xy=0 xz=1 yz=
 }
 
 // ---------------------------------------------------------------- providers
+// lop-sided advection rows: stencil4[offset] of order 4 and the order-2 rows
+// (bam_amr_advection.h:28-36,90-146), row index = offset
+__constant__ double kAdvSt4[5][5] = {{25. / 12, -48. / 12, 36. / 12, -16. / 12, 3. / 12},
+                                     {3. / 12, 10. / 12, -18. / 12, 6. / 12, -1. / 12},
+                                     {-1. / 12, 8. / 12, 0. / 12, -8. / 12, 1. / 12},
+                                     {1. / 12, -6. / 12, 18. / 12, -10. / 12, -3. / 12},
+                                     {-3. / 12, 16. / 12, -36. / 12, 48. / 12, -25. / 12}};
+__constant__ double kAdvSt2[3][3] = {{1.5, -2.0, 0.5}, {0.5, 0.0, -0.5}, {-0.5, 2.0, -1.5}};
+
 // runtime-order provider following the reference per point
 struct Edge {
   long q;
@@ -128,15 +137,10 @@ struct Edge {
     auto row = [&](long o) { return (f[o - 2 * w] - f[o + 2 * w]) + 8.0 * (f[o + w] - f[o - w]); };
     return ((row(q - 2 * t) - row(q + 2 * t)) + 8.0 * (row(q + t) - row(q - t))) * G.i144hh[mixid(a, b)];
   }
-  // advection weights over offsets -4..4 (slot 4 = centre)
+  // advection weights over offsets -4..4 (slot 4 = centre), rows of
+  // set_advection4 / stencil4 (bam_amr_advection.h:28-155)
   double W[3][9];
   __device__ void set_adv(const double beta[3], const Z4cK &K) {
-    const double st4[5][5] = {{25. / 12, -48. / 12, 36. / 12, -16. / 12, 3. / 12},
-                              {3. / 12, 10. / 12, -18. / 12, 6. / 12, -1. / 12},
-                              {-1. / 12, 8. / 12, 0. / 12, -8. / 12, 1. / 12},
-                              {1. / 12, -6. / 12, 18. / 12, -10. / 12, -3. / 12},
-                              {-3. / 12, 16. / 12, -36. / 12, 48. / 12, -25. / 12}};
-    const double st2[3][3] = {{1.5, -2.0, 0.5}, {0.5, 0.0, -0.5}, {-0.5, 2.0, -1.5}};
     for (int a = 0; a < 3; a++) {
       for (int k = 0; k < 9; k++) W[a][k] = 0.0;
       const double wx = beta[a] * G.ih[a];
@@ -155,10 +159,10 @@ struct Edge {
       }
       if (wsign == 0) continue;
       if (orda == 2) {
-        for (int k = 0; k < 3; k++) W[a][4 + wsign * (k - offset)] += -wsign * wx * st2[offset][k];
+        for (int k = 0; k < 3; k++) W[a][4 + wsign * (k - offset)] += -wsign * wx * kAdvSt2[offset][k];
       } else {
         offset += K.lopsided;
-        for (int k = 0; k < 5; k++) W[a][4 + wsign * (k - offset)] += -wsign * wx * st4[offset][k];
+        for (int k = 0; k < 5; k++) W[a][4 + wsign * (k - offset)] += -wsign * wx * kAdvSt4[offset][k];
       }
     }
   }
@@ -593,11 +597,6 @@ __device__ __forceinline__ double t_ko1(const double *T) {
 __device__ __forceinline__ double t_ko(const double *T, const Z4cGeo &G) {
   return -(t_ko1<0>(T) * G.ih[0] + t_ko1<1>(T) * G.ih[1] + t_ko1<2>(T) * G.ih[2]);
 }
-__device__ __forceinline__ void t_grad(const double *T, const Z4cGeo &G, double d[3]) {
-  d[0] = t_d1<0>(T, G);
-  d[1] = t_d1<1>(T, G);
-  d[2] = t_d1<2>(T, G);
-}
 // all six second derivatives (xx xy xz yy yz zz)
 __device__ __forceinline__ void t_hess(const double *T, const Z4cGeo &G, double d[6]) {
   d[0] = t_d2<0>(T, G);
@@ -606,6 +605,87 @@ __device__ __forceinline__ void t_hess(const double *T, const Z4cGeo &G, double 
   d[3] = t_d2<1>(T, G);
   d[4] = t_dd<1, 2>(T, G);
   d[5] = t_d2<2>(T, G);
+}
+
+// ---- stencils of the box-edge shell (bam_amr_loops_box.h:186-215,
+// bam_amr_advection.h:90-146): a point one away from an edge uses order-2
+// centred differences and order-2 advection rows, dissipation needs two or
+// more points to every edge, and the advection row shifts away from an edge
+// closer than rstencil.  The shell instantiation (SH = true) of the tile
+// kernel selects these per point; every selected stencil stays inside the
+// box, hence inside the tile's 3-point halo.
+struct Sten {
+  int o2;               // order-2 stencils (one point from an edge)
+  int ko;               // dissipation on (two or more points from every edge)
+  int shift;            // some advection row differs from the standard lop-sided one
+  int offp[3], offm[3]; // advection row for beta > 0 / beta < 0 per direction
+  double wx[3];         // beta^a / h_a
+};
+
+// neighbour at a run-time offset o along A (|o| <= HALO); T is the thread's
+// tile corner, as for at<>
+template <int A>
+__device__ __forceinline__ double ts(const double *T, int o) {
+  return T[HALO * (1 + SX + SX * SY) + o * (A == 0 ? 1 : (A == 1 ? SX : SX * SY))];
+}
+
+template <bool SH, int A>
+__device__ __forceinline__ double s_d1(const double *T, const Z4cGeo &G, const Sten &S) {
+  if (SH && S.o2) return (ax<A, 1>(T) - ax<A, -1>(T)) * G.i2h[A];
+  return t_d1<A>(T, G);
+}
+template <bool SH>
+__device__ __forceinline__ void s_grad(const double *T, const Z4cGeo &G, const Sten &S, double d[3]) {
+  d[0] = s_d1<SH, 0>(T, G, S);
+  d[1] = s_d1<SH, 1>(T, G, S);
+  d[2] = s_d1<SH, 2>(T, G, S);
+}
+template <int A, int B>
+__device__ __forceinline__ double t_dd2(const double *T, const Z4cGeo &G) {
+  constexpr int AX = (A == 0), AY = (A == 1), AZ = (A == 2), BX = (B == 0), BY = (B == 1), BZ = (B == 2);
+  return ((at<AX + BX, AY + BY, AZ + BZ>(T) - at<AX - BX, AY - BY, AZ - BZ>(T)) -
+          (at<-AX + BX, -AY + BY, -AZ + BZ>(T) - at<-AX - BX, -AY - BY, -AZ - BZ>(T))) *
+         G.i4hh[A + B - 1];
+}
+template <bool SH>
+__device__ __forceinline__ void s_hess(const double *T, const Z4cGeo &G, const Sten &S, double d[6]) {
+  if (SH && S.o2) {
+    d[0] = (ax<0, -1>(T) + ax<0, 1>(T) - 2.0 * ax<0, 0>(T)) * G.ihh[0];
+    d[1] = t_dd2<0, 1>(T, G);
+    d[2] = t_dd2<0, 2>(T, G);
+    d[3] = (ax<1, -1>(T) + ax<1, 1>(T) - 2.0 * ax<1, 0>(T)) * G.ihh[1];
+    d[4] = t_dd2<1, 2>(T, G);
+    d[5] = (ax<2, -1>(T) + ax<2, 1>(T) - 2.0 * ax<2, 0>(T)) * G.ihh[2];
+    return;
+  }
+  t_hess(T, G, d);
+}
+// advection along A with the reference's row selection (Edge::set_adv)
+template <int A>
+__device__ __forceinline__ double s_adv1(const double *T, const Sten &S) {
+  const double wx = S.wx[A];
+  if (wx == 0.0) return 0.0;
+  const int sg = wx > 0.0 ? 1 : -1;
+  const int off = wx > 0.0 ? S.offp[A] : S.offm[A];
+  double acc = 0.0;
+  if (S.o2) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) acc += kAdvSt2[off][k] * ts<A>(T, sg * (k - off));
+  } else {
+#pragma unroll
+    for (int k = 0; k < 5; k++) acc += kAdvSt4[off][k] * ts<A>(T, sg * (k - off));
+  }
+  return -sg * wx * acc;
+}
+template <bool SH>
+__device__ __forceinline__ double s_adv(const double *T, const double pw[3], const double mw[3], const Sten &S) {
+  if (SH && S.shift) return s_adv1<0>(T, S) + s_adv1<1>(T, S) + s_adv1<2>(T, S);
+  return t_adv(T, pw, mw);
+}
+template <bool SH>
+__device__ __forceinline__ double s_ko(const double *T, const Z4cGeo &G, const Sten &S) {
+  if (SH && !S.ko) return 0.0;
+  return t_ko(T, G);
 }
 
 // Staging of one field into the LDS tile, software-pipelined: the global
@@ -621,7 +701,7 @@ __device__ constexpr int STAGE_ORDER[25] = {0,  1,  2,  3,  4,  5,  14, 15, 16, 
 struct Stager {
   int goff[NLOAD];
   unsigned ok;      // bit k: slot k lies inside the tile and the box
-  long cbase;       // linear index of the tile corner (i0-3, j0-3, k0-3)
+  long cbase;       // linear index of the tile corner (i0-3, j0-3, k0-3); may lie before the box
   double pf[NLOAD];
   __device__ Stager(const Z4cGeo &G, int i0, int j0, int k0, int tid) {
     cbase = G.noffset + (i0 - HALO) + G.sj * (j0 - HALO) + G.sk * (k0 - HALO);
@@ -630,15 +710,16 @@ struct Stager {
     for (int k = 0; k < NLOAD; k++) {
       const int idx = tid + k * NTHR;
       const int x = idx % SX, y = (idx / SX) % SY, z = idx / (SX * SY);
-      const bool in = idx < TILE && (i0 - HALO + x) < G.m && (j0 - HALO + y) < G.n && (k0 - HALO + z) < G.o;
+      const int gi = i0 - HALO + x, gj = j0 - HALO + y, gk = k0 - HALO + z;
+      const bool in = idx < TILE && gi >= 0 && gj >= 0 && gk >= 0 && gi < G.m && gj < G.n && gk < G.o;
       goff[k] = in ? (int)(x + G.sj * y + G.sk * z) : 0;
       ok |= in ? (1u << k) : 0u;
     }
   }
+  // slots outside the box are staged as 0; no stencil of a computed point reads them
   __device__ __forceinline__ void load(const double *__restrict__ f) {
-    const double *fb = f + cbase;
 #pragma unroll
-    for (int k = 0; k < NLOAD; k++) pf[k] = (ok >> k & 1u) ? fb[goff[k]] : 0.0;
+    for (int k = 0; k < NLOAD; k++) pf[k] = (ok >> k & 1u) ? f[cbase + goff[k]] : 0.0;
   }
   __device__ __forceinline__ void commit(double *tile, int tid) {
     __syncthreads();
@@ -661,13 +742,47 @@ __device__ __forceinline__ void pin(double (&a)[N]) {
   for (int i = 0; i < N; i++) pin(a[i]);
 }
 
+// SH = false: every point of [lo, hi] is >= 3 from each edge (standard
+// stencils).  SH = true: the box-edge shell; tiles cover [lo, hi], a block
+// whose tile holds no shell point returns at once, and only points closer
+// than 3 to an edge are computed, with the per-point rules of Sten.
+template <bool SH>
 __global__ void __launch_bounds__(NTHR) z4c_rhs_bulk(Z4cPtrs P, Z4cGeo G, Z4cK K, int *flag) {
   __shared__ double tile[TILE];
   const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
   const int tid = tx + TX * (ty + TY * tz);
   const int i0 = G.lo[0] + blockIdx.x * TX, j0 = G.lo[1] + blockIdx.y * TY, k0 = G.lo[2] + blockIdx.z * TZ;
   const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
-  const bool active = (i <= G.hi[0] && j <= G.hi[1] && k <= G.hi[2]);
+  Sten S{};
+  bool active = (i <= G.hi[0] && j <= G.hi[1] && k <= G.hi[2]);
+  if (SH) {
+    // uniform per block: does the tile reach within 3 points of an edge?
+    const int ie = min(i0 + TX - 1, G.hi[0]), je = min(j0 + TY - 1, G.hi[1]), ke = min(k0 + TZ - 1, G.hi[2]);
+    if (i0 >= 3 && j0 >= 3 && k0 >= 3 && ie <= G.m - 4 && je <= G.n - 4 && ke <= G.o - 4) return;
+    // positions past hi (padding threads, never stored) are clamped to hi, so
+    // that their rows and stencil reach stay within the tables and the halo
+    const int pos[3] = {min(i, G.hi[0]), min(j, G.hi[1]), min(k, G.hi[2])};
+    const int pmax[3] = {G.m - 1, G.n - 1, G.o - 1};
+    int dmin = 1 << 30;
+#pragma unroll
+    for (int a = 0; a < 3; a++) dmin = min(dmin, min(pos[a], pmax[a] - pos[a]));
+    active = active && dmin <= 2;
+    S.o2 = dmin == 1;
+    S.ko = dmin >= 2;
+    S.shift = S.o2;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const int dl = pos[a], dh = pmax[a] - pos[a];
+      if (S.o2) {
+        S.offm[a] = dl >= 2 ? 0 : 2 - dl;
+        S.offp[a] = dh >= 2 ? 0 : 2 - dh;
+      } else {
+        S.offm[a] = (dl >= 3 ? 0 : 3 - dl) + 1;
+        S.offp[a] = (dh >= 3 ? 0 : 3 - dh) + 1;
+        S.shift |= (S.offm[a] != 1 || S.offp[a] != 1);
+      }
+    }
+  }
   const long q = G.noffset + i + G.sj * j + G.sk * k;
   const double *T = tile + tx + SX * (ty + SY * tz);
   const double *const *u = P.u;
@@ -711,6 +826,7 @@ __global__ void __launch_bounds__(NTHR) z4c_rhs_bulk(Z4cPtrs P, Z4cGeo G, Z4cK K
     const double w = beta[a] * G.ih[a];
     pw[a] = (w > 0.0 ? w : 0.0) * (1.0 / 12.0);
     mw[a] = (w < 0.0 ? w : 0.0) * (1.0 / 12.0);
+    if (SH) S.wx[a] = w;
   }
   const bool ko = (K.order_dissipation == 4);
   const double eps = ko ? K.dissfactor : 0.0;
@@ -742,13 +858,13 @@ __global__ void __launch_bounds__(NTHR) z4c_rhs_bulk(Z4cPtrs P, Z4cGeo G, Z4cK K
 static_for<6>([&](auto I_) {
     constexpr int ab = decltype(I_)::value;
     advance(ab);
-    dg[0][ab] = t_d1<0>(T, G);
-    dg[1][ab] = t_d1<1>(T, G);
-    dg[2][ab] = t_d1<2>(T, G);
+    dg[0][ab] = s_d1<SH, 0>(T, G, S);
+    dg[1][ab] = s_d1<SH, 1>(T, G, S);
+    dg[2][ab] = s_d1<SH, 2>(T, G, S);
     double h6[6];
-    t_hess(T, G, h6);
+    s_hess<SH>(T, G, S, h6);
     R[ab] = -0.5 * trace(h6);
-    rhs[ab] = t_adv(T, pw, mw) + eps * t_ko(T, G);
+    rhs[ab] = s_adv<SH>(T, pw, mw, S) + eps * s_ko<SH>(T, G, S);
     pin(dg[0][ab]);
     pin(dg[1][ab]);
     pin(dg[2][ab]);
@@ -810,13 +926,13 @@ static_for<3>([&](auto I_) {
     constexpr int c = decltype(I_)::value;
     advance(6 + c);
     double dG[3];
-    t_grad(T, G, dG);
+    s_grad<SH>(T, G, S,dG);
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
       for (int b = a; b < 3; b++) R[sy(a, b)] += 0.5 * (g[sy(c, a)] * dG[b] + g[sy(c, b)] * dG[a]);
-    advG[c] = t_adv(T, pw, mw);
-    koG[c] = eps * t_ko(T, G);
+    advG[c] = s_adv<SH>(T, pw, mw, S);
+    koG[c] = eps * s_ko<SH>(T, G, S);
     pin(R);
     pin(advG[c]);
     pin(koG[c]);
@@ -830,10 +946,10 @@ static_for<3>([&](auto I_) {
   double dchi[3], df[3];
   {
     advance(9);
-    t_grad(T, G, dchi);
+    s_grad<SH>(T, G, S,dchi);
     double h6[6];
-    t_hess(T, G, h6);
-    rhs[6] = t_adv(T, pw, mw) + eps * t_ko(T, G);
+    s_hess<SH>(T, G, S, h6);
+    rhs[6] = s_adv<SH>(T, pw, mw, S) + eps * s_ko<SH>(T, G, S);
 #pragma unroll
     for (int a = 0; a < 3; a++) df[a] = oocp * dchi[a] * oochig;
     double cddf[6];
@@ -864,10 +980,10 @@ static_for<3>([&](auto I_) {
   double da[3], cdda[6];
   {
     advance(10);
-    t_grad(T, G, da);
+    s_grad<SH>(T, G, S,da);
     double h6[6];
-    t_hess(T, G, h6);
-    rhs[18] = K.oplogwithshift * t_adv(T, pw, mw) + eps * t_ko(T, G);
+    s_hess<SH>(T, G, S, h6);
+    rhs[18] = K.oplogwithshift * s_adv<SH>(T, pw, mw, S) + eps * s_ko<SH>(T, G, S);
     const double dfda = contract(df, da);
 #pragma unroll
     for (int a = 0; a < 3; a++)
@@ -887,18 +1003,18 @@ static_for<3>([&](auto I_) {
   double dKT[3], koT;
   {
     advance(11);
-    dKT[0] = 2.0 * t_d1<0>(T, G);
-    dKT[1] = 2.0 * t_d1<1>(T, G);
-    dKT[2] = 2.0 * t_d1<2>(T, G);
-    rhs[13] = t_adv(T, pw, mw) + eps * t_ko(T, G);
+    dKT[0] = 2.0 * s_d1<SH, 0>(T, G, S);
+    dKT[1] = 2.0 * s_d1<SH, 1>(T, G, S);
+    dKT[2] = 2.0 * s_d1<SH, 2>(T, G, S);
+    rhs[13] = s_adv<SH>(T, pw, mw, S) + eps * s_ko<SH>(T, G, S);
     pin(dKT);
     pin(rhs[13]);
     advance(12);
-    dKT[0] += t_d1<0>(T, G);
-    dKT[1] += t_d1<1>(T, G);
-    dKT[2] += t_d1<2>(T, G);
-    rhs[17] = t_adv(T, pw, mw);  // KO of Theta is added after the zeroThetaRHS factor
-    koT = eps * t_ko(T, G);
+    dKT[0] += s_d1<SH, 0>(T, G, S);
+    dKT[1] += s_d1<SH, 1>(T, G, S);
+    dKT[2] += s_d1<SH, 2>(T, G, S);
+    rhs[17] = s_adv<SH>(T, pw, mw, S);  // KO of Theta is added after the zeroThetaRHS factor
+    koT = eps * s_ko<SH>(T, G, S);
     pin(dKT);
     pin(rhs[17]);
     pin(koT);
@@ -908,7 +1024,7 @@ static_for<3>([&](auto I_) {
 static_for<6>([&](auto I_) {
     constexpr int ab = decltype(I_)::value;
     advance(13 + ab);
-    rhs[7 + ab] = t_adv(T, pw, mw) + eps * t_ko(T, G);
+    rhs[7 + ab] = s_adv<SH>(T, pw, mw, S) + eps * s_ko<SH>(T, G, S);
     pin(rhs[7 + ab]);
   });
 
@@ -917,15 +1033,15 @@ static_for<6>([&](auto I_) {
 static_for<3>([&](auto I_) {
     constexpr int c = decltype(I_)::value;
     advance(19 + c);
-    db[0][c] = t_d1<0>(T, G);
-    db[1][c] = t_d1<1>(T, G);
-    db[2][c] = t_d1<2>(T, G);
+    db[0][c] = s_d1<SH, 0>(T, G, S);
+    db[1][c] = s_d1<SH, 1>(T, G, S);
+    db[2][c] = s_d1<SH, 2>(T, G, S);
     double h6[6];
-    t_hess(T, G, h6);
+    s_hess<SH>(T, G, S, h6);
     pG[c] = trace(h6);
 #pragma unroll
     for (int a = 0; a < 3; a++) ootd[a] = (c == 0 ? 0.0 : ootd[a]) + (1.0 / 3.0) * h6[sy(a, c)];
-    rhs[19 + c] = K.withShiftadv * t_adv(T, pw, mw) + eps * t_ko(T, G);
+    rhs[19 + c] = K.withShiftadv * s_adv<SH>(T, pw, mw, S) + eps * s_ko<SH>(T, G, S);
     pin(db[0][c]);
     pin(db[1][c]);
     pin(db[2][c]);
@@ -937,7 +1053,7 @@ static_for<3>([&](auto I_) {
 static_for<3>([&](auto I_) {
     constexpr int c = decltype(I_)::value;
     advance(22 + c);
-    rhs[22 + c] = (K.withB != 0.0 ? K.withBadv * t_adv(T, pw, mw) : 0.0) + eps * t_ko(T, G);
+    rhs[22 + c] = (K.withB != 0.0 ? K.withBadv * s_adv<SH>(T, pw, mw, S) : 0.0) + eps * s_ko<SH>(T, G, S);
     pin(rhs[22 + c]);
   });
 
@@ -1132,13 +1248,21 @@ static int launch_box(const DevBox &B, const Z4cPtrs &P, const Z4cK &K, int *fla
     dim3 grid((bhi[0] - blo[0] + TX) / TX, (bhi[1] - blo[1] + TY) / TY, (bhi[2] - blo[2] + TZ) / TZ);
     {
       TimerScope t("z4_rhs");
-      z4c_rhs_bulk<<<grid, dim3(TX, TY, TZ), 0, stream()>>>(P, Gb, K, flag);
+      z4c_rhs_bulk<false><<<grid, dim3(TX, TY, TZ), 0, stream()>>>(P, Gb, K, flag);
     }
     BH_HIP(hipGetLastError());
   }
   const bool need_edge = !have_bulk || blo[0] > G.lo[0] || bhi[0] < G.hi[0] || blo[1] > G.lo[1] ||
                          bhi[1] < G.hi[1] || blo[2] > G.lo[2] || bhi[2] < G.hi[2];
-  if (need_edge) {
+  if (need_edge && fast) {
+    // the shell of points closer than 3 to an edge: LDS tile kernel with the per-point rules
+    dim3 grid((G.hi[0] - G.lo[0] + TX) / TX, (G.hi[1] - G.lo[1] + TY) / TY, (G.hi[2] - G.lo[2] + TZ) / TZ);
+    {
+      TimerScope t("z4_rhs_edge");
+      z4c_rhs_bulk<true><<<grid, dim3(TX, TY, TZ), 0, stream()>>>(P, G, K, flag);
+    }
+    BH_HIP(hipGetLastError());
+  } else if (need_edge) {
     G.edge_only = have_bulk ? 1 : 0;
     dim3 grid((G.hi[0] - G.lo[0] + 64) / 64, (G.hi[1] - G.lo[1] + 4) / 4, G.hi[2] - G.lo[2] + 1);
     {

@@ -1,19 +1,31 @@
-"""Benchmark of the MI355X Z4c RK4 path.
+"""Benchmark of the MI355X Z4c / FMR / con2prim path (one JSON line).
 
-Workload (the robust-stability configuration on one GPU): one box of 192^3
-interior points on [-8,8]^3 (h = 1/12) with 3 periodic ghost layers
-(198^3 stored points, 25 FP64 grid functions per register), flat space plus
-seeded uniform noise of amplitude 1e-10 on every evolved variable,
-1+log slicing with shift advection, Gamma driver eta = 2, 4th-order
-centred / lop-sided advection finite differences, KO dissipation sigma=0.1,
-dt = h/4.  A step is one full RK4 step (4 RHS evaluations, boundary copies,
-fused stage updates, algebraic constraints) on device-resident data.
+Workloads (synthetic data of the configurations' shapes, FP64 throughout):
 
-Prints one JSON line (metric = grid-point-updates/s on the GPU) with the
-roofline of the dominant kernel (the Z4c RHS bulk kernel, timed with HIP
-events on the library stream during the timed region) and a CPU baseline
-(the repository's OpenMP CPU oracle on a bounded sample of the same
-workload, on the host cores of the GPU machine).
+* fmr (the headline `value`): Berger-Oliger Z4c evolution of a 7-level FMR
+  hierarchy.  Level 0 is one 128^3 box on [-15.875, 15.875]^3 (h0 = 0.25,
+  radiative boundary); levels 1..6 each hold two boxes around a
+  Brill-Lindquist binary (m = 0.5 at x = -3, +3) with seeded uneven sizes in
+  48..96 per direction (clipped so that every box nests in its parent with
+  a 4-cell margin and the two boxes of a level do not overlap).  Precollapsed
+  lapse, zero shift; 1+log / Gamma driver, 4th-order FD with lop-sided
+  advection, KO 0.1, chi floor 1e-5 in the undifferentiated 1/chi terms, dt_l = h_l / 4 with subcycling, time interpolation of
+  order 3, amr_nbuffer 6, order_RP 4.  A step is one coarse step (level 0
+  once, level l 2^l times, each followed by restriction / prolongation with
+  its parent), captured once as a hipGraph and replayed.  Unit: interior
+  grid-point updates per second, sum over levels of 2^l x interior points.
+* box192: one 192^3 periodic box (+3 ghosts), flat space + 1e-10 noise,
+  full RK4 steps; it times the Z4c RHS bulk kernel for the `roofline`.
+* c2p: con2prim on 2^24 points of the con2prim distribution (ideal gas,
+  Gamma 2, tolerance 1e-12): the reference's plain p-root, the bracketed
+  solver from a cold start, and the learned guess (3-64-64-1 network trained
+  on the GPU on 2^22 solved samples) polished by the bracketed solver.
+
+`roofline` is for the dominant kernel, z4c_rhs_bulk: achieved = 5.5e3
+algorithmic FP64 FLOP per point x points per launch / the launch's average
+duration from HIP events on the library stream.  `cpu_baseline` times the
+OpenMP CPU oracle (the restatement used as the parity checker) on a bounded
+sample of the fmr workload's per-box work on the host cores.
 """
 import argparse
 import json
@@ -31,26 +43,154 @@ FLOP_PER_POINT_RHS = 5.5e3      # algorithmic FP64 FLOP per point of one Z4c RHS
 BYTES_PER_POINT_RHS = 400.0     # 25 fields read + 25 written, FP64
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+BINARY = ((0.5, (-3.0, 0.0, 0.0)), (0.5, (3.0, 0.0, 0.0)))
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=192, help="interior points per direction")
+    ap.add_argument("--steps", type=int, default=3, help="timed coarse steps of the fmr workload")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed coarse steps (the second is captured)")
+    ap.add_argument("--box-steps", type=int, default=10)
+    ap.add_argument("--c2p-n", type=int, default=1 << 24)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default="fmr,box192,c2p")
     return ap.parse_args()
 
 
+# ------------------------------------------------------------------ data
+def puncture_fields(x, y, z):
+    """BL binary: chi = psi^-4, precollapsed lapse psi^-2, flat gt, zero rest"""
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    psi = np.ones_like(X)
+    for m, c in BINARY:
+        psi += m / (2 * np.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2))
+    u = np.zeros((25,) + X.shape)
+    u[0] = u[3] = u[5] = 1.0
+    u[6] = psi ** -4
+    u[18] = psi ** -2
+    return u
+
+
+def c3_geometry(make_box, phy, ref, seed=20261203):
+    """boxes per level and the parent of every box (see the module doc)"""
+    rng = np.random.default_rng(seed)
+    n0, H0 = 128, 0.25
+    x0 = -(n0 - 1) / 2 * H0
+    levels = [[dict(m=n0, n=n0, o=n0, x0=x0, y0=x0, z0=x0, h=H0)]]
+    parents = [[0]]
+    for l in range(1, 7):
+        boxes, par = [], []
+        for b, cx in enumerate((-3.0, 3.0)):
+            p = levels[l - 1][b if l > 1 else 0]
+            H = p["h"]
+            size = []
+            for d, key in enumerate(("m", "n", "o")):
+                s = int(rng.integers(24, 49)) * 2
+                s = min(s, 2 * (p[key] - 8))
+                if l == 1 and d == 0:
+                    s = min(s, int(round(6.0 / (H / 2))) // 2 * 2)
+                size.append(s)
+            c0 = []
+            for d, (key, ctr) in enumerate((("m", cx), ("n", 0.0), ("o", 0.0))):
+                org = p["x0" if d == 0 else ("y0" if d == 1 else "z0")]
+                c = int(round((ctr - org) / H - (size[d] - 2) / 4.0))
+                c = max(4, min(c, p[key] - 4 - size[d] // 2))
+                c0.append(c)
+            box = dict(m=size[0], n=size[1], o=size[2], h=H / 2,
+                       x0=p["x0"] + (c0[0] - 0.25) * H, y0=p["y0"] + (c0[1] - 0.25) * H,
+                       z0=p["z0"] + (c0[2] - 0.25) * H)
+            boxes.append(box)
+            par.append(b if l > 1 else 0)
+        # the two boxes of a level must not overlap in x
+        a, bb = boxes
+        gap = bb["x0"] - (a["x0"] + (a["m"] - 1) * a["h"])
+        if gap <= 0:
+            shift = int(np.ceil((-gap + a["h"]) / (2 * a["h"])))
+            bb["x0"] += shift * 2 * a["h"]
+        levels.append(boxes)
+        parents.append(par)
+    out = []
+    for l, bl in enumerate(levels):
+        fl = phy if l == 0 else ref
+        out.append([make_box(b["m"], b["n"], b["o"], b["x0"], b["y0"], b["z0"], b["h"], bflag=(fl,) * 6)
+                    for b in bl])
+    return out, parents
+
+
+def inner_points(boxes):
+    return sum((b.m - 2) * (b.n - 2) * (b.o - 2) for b in boxes)
+
+
+# ------------------------------------------------------------------ fmr
+def run_fmr(args):
+    import baum_amd as B
+    U, UP, UPP, R, V = (list(range(25 * k, 25 * k + 25)) for k in range(5))
+    FP, FR, NV = 125, 126, 127
+    boxes, parents = c3_geometry(B.make_box, B.PHYBOUND, B.REFBOUND)
+    levels = []
+    for l, bl in enumerate(boxes):
+        lv = B.Level(bl, NV, l=l)
+        lv.enable(range(NV))
+        for v in range(NV):
+            lv.fill(v, 0.0)
+        u = np.concatenate([puncture_fields(b.x0 + b.dx * np.arange(b.m), b.y0 + b.dy * np.arange(b.n),
+                                            b.z0 + b.dz * np.arange(b.o)).reshape(25, -1) for b in bl], axis=1)
+        lv.put_state(U, u)
+        del u
+        levels.append(lv)
+    for l in range(1, len(levels)):
+        B.set_rp_flags(levels[l - 1], levels[l], parents[l], 6, FP, FR)
+    # chi floor of the reference's binary-puncture parameter files (z4_chi_div_floor = 1e-5):
+    # without it the 1/chi terms overflow at the punctures on levels >= 4
+    par = B.z4c_params(z4_chi_div_floor=1e-5)
+    dt0 = 0.25 * boxes[0][0].dx
+    H = B.Hierarchy(levels, [par] * len(levels), dt0, U, UP, UPP, R, V, order_RP=4, timeinterpolation=3)
+    H.set_check(0)
+    updates = sum(2 ** l * inner_points(bl) for l, bl in enumerate(boxes))
+    # warm-up: the first coarse step eagerly (reduced-order time interpolation), then capture the next
+    H.advance(0)
+    g = H.capture(0)
+    H.launch(g)
+    for _ in range(max(args.warmup - 2, 0)):
+        H.launch(g)
+    B.synchronize()
+    assert H.nonfinite(reset=False) == 0, "non-finite values in the warm-up steps"
+    B.timer_reset()
+    B.timer_enable(True)
+    B.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        H.launch(g)
+    B.synchronize()
+    t1 = time.perf_counter()
+    B.timer_enable(False)
+    bad = H.nonfinite(reset=False)
+    B.Hierarchy.free_graph(g)
+    # per-kernel split from an eager coarse step (timers pause inside graphs)
+    B.timer_reset()
+    B.timer_enable(True)
+    H.advance(0)
+    B.synchronize()
+    B.timer_enable(False)
+    split = {k: B.timer_get(k)[0] for k in ("z4_rhs", "z4_rhs_edge", "rk4_update", "vladd", "algcon",
+                                            "set_boundary", "prolong", "restrict")}
+    desc = [[(b.m, b.n, b.o) for b in bl] for bl in boxes]
+    for lv in levels:
+        lv.free()
+    el = t1 - t0
+    return dict(value=updates * args.steps / el, ms_per_step=1e3 * el / args.steps, updates_per_step=updates,
+                nonfinite=bool(bad), boxes=desc, eager_split_ms=split)
+
+
+# ------------------------------------------------------------------ box192
 def flat_noise(nstore, g, seed):
     rng = np.random.default_rng(seed)
     u = rng.uniform(-1e-10, 1e-10, (25, nstore, nstore, nstore))
     for c in (0, 3, 5, 6, 18):      # gxx gyy gzz chi alpha
         u[c] += 1.0
-    # periodic consistency of the ghost layers (period = interior)
-    for ax in (1, 2, 3):
+    for ax in (1, 2, 3):            # periodic ghosts (period = interior)
         idx = [slice(None)] * 4
         src = [slice(None)] * 4
         idx[ax] = slice(0, g)
@@ -62,10 +202,9 @@ def flat_noise(nstore, g, seed):
     return u
 
 
-def gpu_run(args):
+def run_box(args):
     import baum_amd as B
-    g = 3
-    n = args.n
+    g, n = 3, 192
     nst = n + 2 * g
     h = 16.0 / n
     dt = 0.25 * h
@@ -80,53 +219,163 @@ def gpu_run(args):
     lev.put_state(U, u0.reshape(25, -1))
     del u0
     par = B.z4c_params()
-    for _ in range(args.warmup):
-        lev.z4c_rk4_step(par, dt, U, P, R, V, pgh=g)
+    lev.z4c_rk4_step(par, dt, U, P, R, V, pgh=g)
     B.synchronize()
     B.timer_reset()
     B.timer_enable(True)
-    B.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(args.box_steps):
         lev.z4c_rk4_step(par, dt, U, P, R, V, pgh=g)
     B.synchronize()
     t1 = time.perf_counter()
     B.timer_enable(False)
-    ms_rhs, n_rhs = B.timer_get("z4_rhs")
-    ms_upd, n_upd = B.timer_get("rk4_update")
-    ms_bc, n_bc = B.timer_get("set_boundary")
-    ms_alg, n_alg = B.timer_get("algcon")
-    ms_cpy, n_cpy = B.timer_get("vladd")
+    t = {k: B.timer_get(k) for k in ("z4_rhs", "rk4_update", "set_boundary", "algcon", "vladd")}
     l2 = lev.z4c_ham_l2(U)
     lev.free()
-    return dict(n=n, h=h, dt=dt, elapsed=t1 - t0, ms_rhs=ms_rhs, n_rhs=n_rhs, ms_upd=ms_upd, n_upd=n_upd,
-                ms_bc=ms_bc, ms_alg=ms_alg, ms_cpy=ms_cpy, ham_l2=l2)
+    npts = n ** 3
+    launch_ms = t["z4_rhs"][0] / max(t["z4_rhs"][1], 1)
+    return dict(value=npts * args.box_steps / (t1 - t0), ms_per_step=1e3 * (t1 - t0) / args.box_steps,
+                rhs_launch_ms=launch_ms, npts=npts, ham_l2_after=l2,
+                breakdown_ms_per_step={k: v[0] / args.box_steps for k, v in t.items()})
 
 
+# ------------------------------------------------------------------ c2p
+def c2p_prims(n, seed):
+    """rho log-uniform [1e-10, 1e-2], eps log-uniform [1e-6, 1], W uniform
+    [1, 10] in a random direction, metric flat + 1e-3 seeded perturbation"""
+    rng = np.random.default_rng(seed)
+    rho = 10.0 ** rng.uniform(-10, -2, n)
+    eps = 10.0 ** rng.uniform(-6, 0, n)
+    W = rng.uniform(1.0, 10.0, n)
+    d = rng.standard_normal((3, n))
+    d /= np.linalg.norm(d, axis=0)
+    gm = np.zeros((6, n))
+    gm[0] = gm[3] = gm[5] = 1.0
+    gm += 1e-3 * rng.uniform(-1, 1, (6, n))
+    gdd = (gm[0] * d[0] ** 2 + gm[3] * d[1] ** 2 + gm[5] * d[2] ** 2
+           + 2 * (gm[1] * d[0] * d[1] + gm[2] * d[0] * d[2] + gm[4] * d[1] * d[2]))
+    v = np.sqrt((1.0 - 1.0 / W ** 2) / gdd) * d
+    return np.array([rho, eps, v[0], v[1], v[2]]), gm
+
+
+def run_c2p(args):
+    import baum_amd as B
+    P = B.c2p_params(errormax=1e-12, countmax=100, rhoatm=1e-20)
+
+    def cons_of(n, seed):
+        prim5, gm = c2p_prims(n, seed)
+        dp, dg = B.DeviceArray.from_host(prim5), B.DeviceArray.from_host(gm)
+        dc = B.DeviceArray((5, n))
+        B.p2c_batch(P, dp, dg, dc)
+        dp.free()
+        return dc, dg, prim5[0]
+
+    def timed(fn, reps=3):
+        best = 1e30
+        for _ in range(reps):
+            B.synchronize()
+            B.timer_reset()
+            B.timer_enable(True)
+            fn()
+            B.synchronize()
+            B.timer_enable(False)
+            best = min(best, B.timer_get("c2p")[0])
+        return best
+
+    n = args.c2p_n
+    dc0, dg, rho = cons_of(n, 4242)
+    cons0 = dc0.get()
+    dc = B.DeviceArray((5, n))
+    dpr = B.DeviceArray((8, n))
+    ds, di = B.DeviceArray((n,), np.int32), B.DeviceArray((n,), np.int32)
+    out = {"n": n, "tolerance": 1e-12}
+
+    def reset():
+        dc.put(cons0)
+        pr = np.zeros((8, n))
+        pr[0] = rho * 0.9    # density guess of the reference cold branch
+        dpr.put(pr)
+
+    for name, method in (("plain", B.C2P_PLAIN), ("bracketed", B.C2P_BRACKETED)):
+        def go():
+            reset()
+            B.c2p_batch(P, method, dc, dg, dpr, ds, di)
+        ms = timed(go)
+        st, it = ds.get(), di.get()
+        out[name] = dict(ms=ms, pts_per_s=n / (ms * 1e-3), converged=float(np.mean(st == B.C2P_OK)),
+                         mean_iters=float(it.mean()), max_iters=int(it.max()),
+                         iter_hist=np.bincount(np.minimum(it, 12), minlength=13).tolist())
+    p_brk = dpr.get()[5]
+    # learned guess: training data solved by the bracketed solver, training on the device
+    ntr = 1 << 22
+    tc, tg, _ = cons_of(ntr, 777)
+    tp = B.DeviceArray.from_host(np.zeros((8, ntr)))
+    ts, ti = B.DeviceArray((ntr,), np.int32), B.DeviceArray((ntr,), np.int32)
+    B.c2p_batch(P, B.C2P_BRACKETED, tc, tg, tp, ts, ti)
+    feat, tgt = B.DeviceArray((ntr, 3), np.float32), B.DeviceArray((ntr,), np.float32)
+    B.mlp_prepare(P, tc, tg, tp, ts, feat, tgt)
+    f, t = feat.get(), tgt.get()
+    ok = np.isfinite(t)
+    mu, sd = f[ok].mean(axis=0), f[ok].std(axis=0)
+    nrm = np.array([mu[0], mu[1], mu[2], 1 / sd[0], 1 / sd[1], 1 / sd[2], t[ok].mean(), t[ok].std()], np.float32)
+    net = B.Mlp(seed=11)
+    net.set(None, nrm)
+    B.synchronize()
+    t0 = time.perf_counter()
+    losses = net.train(feat, tgt, epochs=30, batch=4096, lr=3e-3, seed=12)
+    B.synchronize()
+    train_s = time.perf_counter() - t0
+    for a in (tc, tg, tp, ts, ti, feat, tgt):
+        a.free()
+    fb = B.DeviceArray.from_host(np.zeros(1, np.int32))
+
+    def go_l():
+        reset()
+        fb.put(np.zeros(1, np.int32))
+        B.c2p_learned_batch(P, net, dc, dg, dpr, ds, di, fb, polish_max=8)
+    ms = timed(go_l)
+    st, it, pl = ds.get(), di.get(), dpr.get()[5]
+    agree = float(np.max(np.abs(pl / p_brk - 1)))
+    out["learned"] = dict(ms=ms, pts_per_s=n / (ms * 1e-3), converged=float(np.mean(st == B.C2P_OK)),
+                          mean_iters=float(it.mean()), max_iters=int(it.max()),
+                          iter_hist=np.bincount(np.minimum(it, 12), minlength=13).tolist(),
+                          fallback_fraction=float(fb.get()[0]) / n, max_rel_diff_p_vs_bracketed=agree,
+                          training=dict(samples=ntr, epochs=30, batch=4096, seconds=train_s,
+                                        loss_first=float(losses[0]), loss_last=float(losses[-1])))
+    out["learned_vs_plain_speedup"] = out["learned"]["pts_per_s"] / out["plain"]["pts_per_s"]
+    out["learned_vs_bracketed_speedup"] = out["learned"]["pts_per_s"] / out["bracketed"]["pts_per_s"]
+    for a in (dc0, dg, dc, dpr, ds, di, fb):
+        a.free()
+    net.free()
+    return out
+
+
+# ------------------------------------------------------------------ CPU baseline
 def cpu_baseline(target_s):
     """the OpenMP CPU oracle (test infrastructure, used here only as the
-    reported baseline) on a 48^3-interior periodic noise box"""
+    reported baseline): RK4 steps of one 48^3-interior box of the fmr
+    workload's level-3 data (binary puncture, radiative boundary)"""
     import oracle as O
-    g = 3
-    n = 48
-    nst = n + 2 * g
-    h = 16.0 / 192
-    ob = O.make_box(nst, nst, nst, 0.0, 0.0, 0.0, h, skip=(g,) * 6)
-    u = flat_noise(nst, g, seed=7)
+    n = 50
+    h = 0.25 / 8
+    x0 = -3.0 - (n - 1) / 2 * h
+    ob = O.make_box(n, n, n, x0, -(n - 1) / 2 * h, -(n - 1) / 2 * h, h)
+    u = puncture_fields(x0 + h * np.arange(n), -(n - 1) / 2 * h + h * np.arange(n), -(n - 1) / 2 * h + h * np.arange(n))
     p = O.z4c_params()
-    work = np.zeros((75, nst, nst, nst))
+    work = np.zeros((75, n, n, n))
     steps = 0
     t0 = time.perf_counter()
     while True:
-        O.z4c_rk4_step(ob, p, 0.25 * h, u, bc=2, pgh=g, work=work)
+        O.z4c_rk4_step(ob, p, 0.25 * h, u, bc=1, pgh=0, work=work)
         steps += 1
         el = time.perf_counter() - t0
-        if el >= target_s or steps >= 200:
+        if el >= target_s or steps >= 500:
             break
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=steps * n ** 3 / el, unit="grid-point-updates/s", cores=cores, kind="port",
-                sample="%d RK4 steps of the same Z4c configuration on a 48^3-interior periodic box "
-                       "(OpenMP C oracle), %.1f s" % (steps, el))
+    return dict(value=steps * (n - 2) ** 3 / el, unit="grid-point-updates/s", cores=cores, kind="port",
+                sample="%d RK4 steps of one 48^3-interior box of the FMR workload (binary puncture data, "
+                       "radiative boundary) with the OpenMP C oracle, %.1f s; restriction/prolongation "
+                       "(7%% of the reference's FMR time) not included" % (steps, el))
 
 
 def traffic_from_profile():
@@ -143,30 +392,40 @@ def traffic_from_profile():
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (the multi-GPU box decomposition is a design "
-              "described in DESIGN.md, not an implemented path); use --gpus 1", file=sys.stderr)
+        print("bench.py: this build runs on one GPU only (the multi-GPU decomposition is a design, not an "
+              "implemented path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
-    r = gpu_run(args)
-    npts = r["n"] ** 3
-    value = npts * args.steps / r["elapsed"]
-    launch_ms = r["ms_rhs"] / max(r["n_rhs"], 1)
-    flops = FLOP_PER_POINT_RHS * npts
-    achieved = flops / (launch_ms * 1e-3) / 1e12
-    roof = {"bound": "fp64", "kernel": "z4c_rhs_bulk", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profile(),
-            "launch_ms": launch_ms, "algorithmic_flop_per_launch": flops,
-            "hbm_gbs_algorithmic": BYTES_PER_POINT_RHS * npts / (launch_ms * 1e-3) / 1e9}
-    out = {"metric": METRIC, "value": value, "unit": "grid-point-updates/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": 1e3 * r["elapsed"] / args.steps, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": "Z4c RK4, one %d^3 periodic box (+3 ghosts), flat + 1e-10 noise, "
-                                  "1+log / Gamma driver, 4th-order FD, KO 0.1, dt=h/4" % r["n"],
-                      "points_per_step": npts, "parallelism": "single GPU"},
-           "roofline": roof,
-           "breakdown_ms_per_step": {"z4c_rhs": r["ms_rhs"] / args.steps, "rk4_update": r["ms_upd"] / args.steps,
-                                     "set_boundary": r["ms_bc"] / args.steps, "algcon": r["ms_alg"] / args.steps,
-                                     "copy_up": r["ms_cpy"] / args.steps},
-           "ham_l2_after": r["ham_l2"]}
+    only = set(args.only.split(","))
+    import baum_amd as B
+    B.init()
+    out = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
+    fmr = run_fmr(args) if "fmr" in only else None
+    box = run_box(args) if "box192" in only else None
+    if fmr:
+        out.update(value=fmr["value"], unit="grid-point-updates/s", ms_per_step=fmr["ms_per_step"])
+        out["config"] = {"workload": "Z4c BO evolution of a 7-level FMR hierarchy: 128^3 level 0 + 2 boxes of "
+                                     "48..96^3 per level 1..6 around a BL binary (m=0.5 at x=+-3), nbuffer 6, "
+                                     "order_RP 4, time interpolation 3, z4_chi_div_floor 1e-5, KO 0.1, "
+                                     "one hipGraph per coarse step",
+                         "updates_per_coarse_step": fmr["updates_per_step"], "boxes": fmr["boxes"],
+                         "parallelism": "single GPU"}
+        out["fmr"] = {k: fmr[k] for k in ("nonfinite", "eager_split_ms")}
+    elif box:
+        out.update(value=box["value"], unit="grid-point-updates/s", ms_per_step=box["ms_per_step"])
+        out["config"] = {"workload": "Z4c RK4, one 192^3 periodic box", "parallelism": "single GPU"}
+    if box:
+        flops = FLOP_PER_POINT_RHS * box["npts"]
+        achieved = flops / (box["rhs_launch_ms"] * 1e-3) / 1e12
+        out["roofline"] = {"bound": "fp64", "kernel": "z4c_rhs_bulk", "achieved": achieved,
+                           "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                           "traffic": traffic_from_profile(), "launch_ms": box["rhs_launch_ms"],
+                           "algorithmic_flop_per_launch": flops,
+                           "workload": "192^3 periodic box (+3 ghosts), one RHS launch"}
+        out["box192"] = {"value": box["value"], "unit": "grid-point-updates/s", "ms_per_step": box["ms_per_step"],
+                         "breakdown_ms_per_step": box["breakdown_ms_per_step"], "ham_l2_after": box["ham_l2_after"]}
+    if "c2p" in only:
+        out["c2p"] = run_c2p(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))
