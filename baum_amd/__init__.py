@@ -93,12 +93,19 @@ def lib():
     _sig(L, "bamhip_level_set_check", i, [vp, i])
     _sig(L, "bamhip_level_nonfinite", i, [vp, i, IP])
     _sig(L, "bamhip_z4c_rhs", i, [vp, C.POINTER(Z4cParams), IP, IP, d, IP])
+    _sig(L, "bamhip_z4c_tune_mode", i, [i])
+    _sig(L, "bamhip_z4c_tune_clear", i, [])
+    _sig(L, "bamhip_z4c_variant_count", i, [i])
+    _sig(L, "bamhip_z4c_variant_name", C.c_char_p, [i, i])
+    _sig(L, "bamhip_z4c_force_variant", i, [i, i])
+    _sig(L, "bamhip_z4c_tune_report", i, [C.c_char_p, i])
     _sig(L, "bamhip_z4c_set_boundary", i, [vp, C.POINTER(Z4cParams), IP, IP, d, IP, i])
     _sig(L, "bamhip_vladd", i, [vp, i, IP, d, IP, d, IP])
     _sig(L, "bamhip_rk4_stage_update", i, [vp, i, IP, IP, IP, IP, d, d])
     _sig(L, "bamhip_z4c_rk4_step", i, [vp, C.POINTER(Z4cParams), d, IP, IP, IP, IP, i, i])
     _sig(L, "bamhip_z4c_algcon", i, [vp, IP, IP, i, i])
     _sig(L, "bamhip_z4c_ham_l2", i, [vp, IP, d, i, DP])
+    _sig(L, "bamhip_z4c_constraints_l2", i, [vp, IP, d, i, IP, DP])
     _sig(L, "bamhip_set_rp_flags", i, [vp, vp, IP, i, i, i])
     _sig(L, "bamhip_restrict_varlist", i, [vp, vp, i, IP, IP, i, i])
     _sig(L, "bamhip_prolong_varlist", i, [vp, vp, i, IP, IP, i, i])
@@ -288,6 +295,15 @@ class Level:
         out = C.c_double()
         check(lib().bamhip_z4c_ham_l2(self.h, _iarr(u), chipsipower, ham, C.byref(out)), "bamhip_z4c_ham_l2")
         return out.value
+
+    def z4c_constraints_l2(self, u, chipsipower=-4.0, ham=-1, mom=None):
+        """(|ham|, |mom_x|, |mom_y|, |mom_z|) over the inner points
+        (adm.c:14-74); ham / mom: optional variable indices for the fields"""
+        out = (C.c_double * 4)()
+        check(lib().bamhip_z4c_constraints_l2(self.h, _iarr(u), chipsipower, ham,
+                                              _iarr(mom) if mom is not None else None, out),
+              "bamhip_z4c_constraints_l2")
+        return np.array(out[:])
 
     # -- Berger-Oliger time interpolation (evolve.c:298-340, 367-380)
     def bo_interp_begin(self, timeinterpolation, iteration, upp, uc, up=None, up2=None, up3=None, up4=None):
@@ -589,6 +605,39 @@ def timer_get(name):
     n = C.c_long()
     check(lib().bamhip_timer_get(name.encode(), C.byref(ms), C.byref(n)), "bamhip_timer_get")
     return ms.value, n.value
+
+
+# ------------------------------------------------------- RHS kernel tuner
+BULK, SHELL = 0, 1
+
+
+def tune_mode(mode):
+    """0: default tile kernels; 1: time every variant on each new box set"""
+    rc = lib().bamhip_z4c_tune_mode(mode)
+    check(min(rc, 0), "bamhip_z4c_tune_mode")
+    return rc
+
+
+def tune_clear():
+    check(lib().bamhip_z4c_tune_clear(), "bamhip_z4c_tune_clear")
+
+
+def variants(kind):
+    n = lib().bamhip_z4c_variant_count(kind)
+    check(min(n, 0), "bamhip_z4c_variant_count")
+    return [lib().bamhip_z4c_variant_name(kind, v).decode() for v in range(n)]
+
+
+def force_variant(kind, idx):
+    """pin variant idx of kind (BULK, SHELL); -1 hands the choice back to the tuner"""
+    check(lib().bamhip_z4c_force_variant(kind, idx), "bamhip_z4c_force_variant")
+
+
+def tune_report():
+    n = lib().bamhip_z4c_tune_report(None, 0)
+    buf = C.create_string_buffer(n)
+    lib().bamhip_z4c_tune_report(buf, n)
+    return buf.value.decode()
 
 
 def gpu_available():

@@ -274,30 +274,37 @@ __device__ __forceinline__ int syi(int a, int b) {
   return lo * (5 - lo) / 2 + hi;
 }
 
+// Hamiltonian and momentum constraints (adm_constraints_N.m:14-116, usepsi
+// off, no matter) at the inner points of one box; per block the sums of H^2,
+// mom_x^2, mom_y^2, mom_z^2 and the point count go to partial[5 * block].
+struct MomPtrs {
+  double *p[3];
+};
 __global__ void __launch_bounds__(256) ham_kernel(const double *__restrict__ adm, long n, DevBox B, double *ham,
-                                                  double *partial) {
-  __shared__ double red[256];
-  __shared__ double cnt[256];
+                                                  MomPtrs mom, double *partial) {
+  __shared__ double red[5][256];
   const int M = B.m, N = B.n, O = B.o;
   const int i = B.skip[0] + blockIdx.x * 64 + threadIdx.x;
   const int j = B.skip[2] + blockIdx.y * 4 + threadIdx.y;
   const int k = B.skip[4] + blockIdx.z;
   const int tid = threadIdx.x + 64 * threadIdx.y;
-  double h2 = 0.0, c = 0.0;
+  double sums[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (i <= M - 1 - B.skip[1] && j <= N - 1 - B.skip[3] && k <= O - 1 - B.skip[5]) {
     const long st[3] = {1, M, (long)M * N};
     const double h[3] = {B.dx, B.dy, B.dz};
     const long q = B.noffset + i + st[1] * j + st[2] * k;
     const bool o2 = (i == 1 || j == 1 || k == 1 || i == M - 2 || j == N - 2 || k == O - 2);
-    double g[6], Kd[6], dg[3][6], ddg[6][6];
+    double g[6], Kd[6], dg[3][6], ddg[6][6], dK[3][6];
     for (int c6 = 0; c6 < 6; c6++) {
-      const double *f = adm + c6 * n;
+      const double *f = adm + c6 * n, *kf = adm + (6 + c6) * n;
       g[c6] = f[q];
-      Kd[c6] = adm[(6 + c6) * n + q];
+      Kd[c6] = kf[q];
       for (int e = 0; e < 3; e++) {
         const long s = st[e];
         dg[e][c6] = o2 ? (f[q + s] - f[q - s]) / (2 * h[e])
                        : (f[q - 2 * s] - 8 * f[q - s] + 8 * f[q + s] - f[q + 2 * s]) / (12 * h[e]);
+        dK[e][c6] = o2 ? (kf[q + s] - kf[q - s]) / (2 * h[e])
+                       : (kf[q - 2 * s] - 8 * kf[q - s] + 8 * kf[q + s] - kf[q + 2 * s]) / (12 * h[e]);
       }
       for (int e = 0; e < 3; e++)
         for (int e2 = e; e2 < 3; e2++) {
@@ -357,24 +364,49 @@ __global__ void __launch_bounds__(256) ham_kernel(const double *__restrict__ adm
     for (int a = 0; a < 3; a++)
       for (int b = 0; b < 3; b++) KK += Kud[a][b] * Kud[b][a];
     const double H = R + trK * trK - KK;
+    // D_a K_bc = d_a K_bc - Gamma^d_ab K_dc - Gamma^d_ac K_bd, raised index
+    // D^a K_bc = g^ad D_d K_bc; mom^a = g^ab D^c K_bc - g^bc D^a K_bc
+    double cdK[3][6];
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++)
+        for (int cc = b; cc < 3; cc++) {
+          double t = dK[a][syi(b, cc)];
+          for (int d = 0; d < 3; d++) t -= gam[d][syi(a, b)] * Kd[syi(d, cc)] + gam[d][syi(a, cc)] * Kd[syi(b, d)];
+          cdK[a][syi(b, cc)] = t;
+        }
+    double momu[3];
+    for (int a = 0; a < 3; a++) {
+      double t = 0.0;
+      for (int b = 0; b < 3; b++)
+        for (int cc = 0; cc < 3; cc++) {
+          double up1 = 0.0, up2 = 0.0;  // D^c K_bc and D^a K_bc
+          for (int d = 0; d < 3; d++) {
+            up1 += gi[syi(cc, d)] * cdK[d][syi(b, cc)];
+            up2 += gi[syi(a, d)] * cdK[d][syi(b, cc)];
+          }
+          t += gi[syi(a, b)] * up1 - gi[syi(b, cc)] * up2;
+        }
+      momu[a] = t;
+    }
     if (ham) ham[q] = H;
-    h2 = H * H;
-    c = 1.0;
+    sums[0] = H * H;
+    for (int a = 0; a < 3; a++) {
+      const double Ma = g[syi(a, 0)] * momu[0] + g[syi(a, 1)] * momu[1] + g[syi(a, 2)] * momu[2];
+      if (mom.p[a]) mom.p[a][q] = Ma;
+      sums[1 + a] = Ma * Ma;
+    }
+    sums[4] = 1.0;
   }
-  red[tid] = h2;
-  cnt[tid] = c;
+  for (int v = 0; v < 5; v++) red[v][tid] = sums[v];
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if (tid < s) {
-      red[tid] += red[tid + s];
-      cnt[tid] += cnt[tid + s];
-    }
+    if (tid < s)
+      for (int v = 0; v < 5; v++) red[v][tid] += red[v][tid + s];
     __syncthreads();
   }
   if (tid == 0) {
     const long bid = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
-    partial[2 * bid] = red[0];
-    partial[2 * bid + 1] = cnt[0];
+    for (int v = 0; v < 5; v++) partial[5 * bid + v] = red[v][0];
   }
 }
 
@@ -500,41 +532,55 @@ int bamhip_rk4_stage_update(bamhip_level *lev, int n, const int *uc, const int *
   return BAMHIP_OK;
 }
 
-int bamhip_z4c_ham_l2(bamhip_level *lev, const int *u, double chipsipower, int ham, double *l2) {
-  if (!lev || !l2) return set_error(BAMHIP_EINVAL, "bamhip_z4c_ham_l2: NULL argument");
+int bamhip_z4c_constraints_l2(bamhip_level *lev, const int *u, double chipsipower, int ham, const int *mom,
+                              double *l2) {
+  if (!lev || !l2) return set_error(BAMHIP_EINVAL, "bamhip_z4c_constraints_l2: NULL argument");
   Ptrs25 U;
   int e;
-  if ((e = get25(lev, u, 25, U, "bamhip_z4c_ham_l2(u)"))) return e;
+  if ((e = get25(lev, u, 25, U, "bamhip_z4c_constraints_l2(u)"))) return e;
   double *hp = nullptr;
   if (ham >= 0) {
     hp = bamhip_level_ptr(lev, ham);
-    if (!hp) return set_error(BAMHIP_EINVAL, "bamhip_z4c_ham_l2: ham variable %d not enabled", ham);
+    if (!hp) return set_error(BAMHIP_EINVAL, "bamhip_z4c_constraints_l2: ham variable %d not enabled", ham);
   }
+  MomPtrs mp{{nullptr, nullptr, nullptr}};
+  if (mom)
+    for (int a = 0; a < 3; a++) {
+      mp.p[a] = bamhip_level_ptr(lev, mom[a]);
+      if (!mp.p[a])
+        return set_error(BAMHIP_EINVAL, "bamhip_z4c_constraints_l2: mom variable %d not enabled", mom[a]);
+    }
   double *adm = nullptr;
   BH_HIP(hipMallocAsync((void **)&adm, sizeof(double) * 12 * lev->npoints, stream()));
   z4_to_adm_kernel<<<stream_grid(lev->npoints, 1).x, 256, 0, stream()>>>(U, adm, lev->npoints, chipsipower);
-  double sum = 0.0, count = 0.0;
+  double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (const DevBox &B : lev->boxes) {
     const int ni = B.m - B.skip[0] - B.skip[1], nj = B.n - B.skip[2] - B.skip[3], nk = B.o - B.skip[4] - B.skip[5];
     if (ni <= 0 || nj <= 0 || nk <= 0) continue;
     dim3 grid((ni + 63) / 64, (nj + 3) / 4, nk);
     const long nb = (long)grid.x * grid.y * grid.z;
     double *partial = nullptr;
-    BH_HIP(hipMallocAsync((void **)&partial, sizeof(double) * 2 * nb, stream()));
-    ham_kernel<<<grid, dim3(64, 4, 1), 0, stream()>>>(adm, lev->npoints, B, hp, partial);
-    std::vector<double> hpart(2 * nb);
-    BH_HIP(hipMemcpyAsync(hpart.data(), partial, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, stream()));
+    BH_HIP(hipMallocAsync((void **)&partial, sizeof(double) * 5 * nb, stream()));
+    ham_kernel<<<grid, dim3(64, 4, 1), 0, stream()>>>(adm, lev->npoints, B, hp, mp, partial);
+    std::vector<double> hpart(5 * nb);
+    BH_HIP(hipMemcpyAsync(hpart.data(), partial, sizeof(double) * 5 * nb, hipMemcpyDeviceToHost, stream()));
     BH_HIP(hipStreamSynchronize(stream()));
     BH_HIP(hipFreeAsync(partial, stream()));
-    for (long b = 0; b < nb; b++) {
-      sum += hpart[2 * b];
-      count += hpart[2 * b + 1];
-    }
+    for (long b = 0; b < nb; b++)
+      for (int v = 0; v < 5; v++) sum[v] += hpart[5 * b + v];
   }
   BH_HIP(hipFreeAsync(adm, stream()));
   BH_HIP(hipStreamSynchronize(stream()));
-  *l2 = count > 0 ? sqrt(sum / count) : 0.0;
+  for (int v = 0; v < 4; v++) l2[v] = sum[4] > 0 ? sqrt(sum[v] / sum[4]) : 0.0;
   return BAMHIP_OK;
+}
+
+int bamhip_z4c_ham_l2(bamhip_level *lev, const int *u, double chipsipower, int ham, double *l2) {
+  if (!l2) return set_error(BAMHIP_EINVAL, "bamhip_z4c_ham_l2: NULL argument");
+  double all[4];
+  const int e = bamhip_z4c_constraints_l2(lev, u, chipsipower, ham, nullptr, all);
+  if (e == BAMHIP_OK) *l2 = all[0];
+  return e;
 }
 
 }  // extern "C"

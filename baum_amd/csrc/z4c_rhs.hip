@@ -23,10 +23,23 @@
 //   Edge  : any point, with the reference's per-point order selection,
 //           used for the shell of points 1 and 2 away from the box edge and
 //           for parameter choices other than (4, 4, lop-sided 1).
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
 
 #include "bamhip_internal.h"
-#include <utility>
+
+// Contraction of a*b + c into an FMA only within one source expression: the
+// fused operations are then fixed by the source, not by per-kernel backend
+// choices, so every instantiation of the tile kernel (tile shape, staging,
+// register budget) rounds identically and gives bitwise the same RHS.
+#pragma clang fp contract(on)
 
 // compile-time loop: f(integral_constant<int, I>) for I = 0..N-1; used for
 // loops whose body holds a barrier, which the compiler does not unroll, so
@@ -537,68 +550,89 @@ __device__ __forceinline__ bool z4c_point(D &dv, const Z4cPtrs &P, const Z4cK &K
 }
 
 // ------------------------------------------------------- bulk kernel (LDS)
-// Output tile of TX x TY x TZ points per block, one point per thread.  Each
-// of the 25 grid functions is staged in turn into an LDS tile with a halo of
-// 3 points (the reach of the lop-sided advection stencil), so every stencil
-// neighbour is an immediate-offset ds_read_b64 from one per-thread LDS
-// address.  The algebra is ordered by staged field: each field's
-// derivatives, advection and dissipation are consumed as soon as the field
-// is in LDS.
-constexpr int TX = 16, TY = 4, TZ = 4, HALO = 3;
-constexpr int SX = TX + 2 * HALO, SY = TY + 2 * HALO, SZ = TZ + 2 * HALO;
-constexpr int TILE = SX * SY * SZ;
-constexpr int NTHR = TX * TY * TZ;
+// Output tile of TX x TY x TZ points per block (256 threads, one point per
+// thread).  Each of the 25 grid functions is staged in turn into an LDS tile
+// with a halo of 3 points (the reach of the lop-sided advection stencil), so
+// every stencil neighbour is an immediate-offset ds_read_b64 from one
+// per-thread LDS address.  The algebra is ordered by staged field: each
+// field's derivatives, advection and dissipation are consumed as soon as the
+// field is in LDS.  The bulk uses 16 x 4 x 4 tiles (long x rows for
+// coalesced loads); the box-edge shell, two points thick, uses slabs of
+// 2 x 16 x 8, 16 x 2 x 8 and 16 x 8 x 2 on the x, y and z faces.
+constexpr int HALO = 3;
+constexpr int NTHR = 256;
+template <int TX_, int TY_, int TZ_>
+struct Lay {
+  static constexpr int TX = TX_, TY = TY_, TZ = TZ_;
+  static constexpr int SX = TX + 2 * HALO, SY = TY + 2 * HALO, SZ = TZ + 2 * HALO;
+  static constexpr int TILE = SX * SY * SZ;
+  static constexpr int NLOAD = (TILE + NTHR - 1) / NTHR;
+  static_assert(TX * TY * TZ == NTHR, "one point per thread");
+};
+using LBulk = Lay<16, 4, 4>;
+using LSlabX = Lay<2, 16, 8>;
+using LSlabY = Lay<16, 2, 8>;
+using LSlabZ = Lay<16, 8, 2>;
 
-template <int DX, int DY, int DZ>
-__device__ __forceinline__ double at(const double *T) {
-  return T[(DX + HALO) + SX * ((DY + HALO) + SY * (DZ + HALO))];
+// a thread's tile corner in the LDS tile of layout L (its point is at +HALO)
+template <class L>
+struct TP {
+  const double *p;
+};
+
+template <int DX, int DY, int DZ, class L>
+__device__ __forceinline__ double at(TP<L> T) {
+  return T.p[(DX + HALO) + L::SX * ((DY + HALO) + L::SY * (DZ + HALO))];
 }
 // neighbour at offset o along direction A
-template <int A, int O>
-__device__ __forceinline__ double ax(const double *T) {
+template <int A, int O, class L>
+__device__ __forceinline__ double ax(TP<L> T) {
   return at<(A == 0 ? O : 0), (A == 1 ? O : 0), (A == 2 ? O : 0)>(T);
 }
-template <int A>
-__device__ __forceinline__ double t_d1(const double *T, const Z4cGeo &G) {
+template <int A, class L>
+__device__ __forceinline__ double t_d1(TP<L> T, const Z4cGeo &G) {
   return ((ax<A, -2>(T) - ax<A, 2>(T)) + 8.0 * (ax<A, 1>(T) - ax<A, -1>(T))) * G.i12h[A];
 }
-template <int A>
-__device__ __forceinline__ double t_d2(const double *T, const Z4cGeo &G) {
+template <int A, class L>
+__device__ __forceinline__ double t_d2(TP<L> T, const Z4cGeo &G) {
   return (16.0 * (ax<A, -1>(T) + ax<A, 1>(T)) - (ax<A, -2>(T) + ax<A, 2>(T)) - 30.0 * ax<A, 0>(T)) * G.i12hh[A];
 }
 // mixed derivative d_A d_B (A < B): tensor product of 4th-order first differences
-template <int A, int B, int OA>
-__device__ __forceinline__ double t_row(const double *T) {
+template <int A, int B, int OA, class L>
+__device__ __forceinline__ double t_row(TP<L> T) {
   constexpr int X0 = (A == 0 ? OA : 0), Y0 = (A == 1 ? OA : 0), Z0 = (A == 2 ? OA : 0);
   constexpr int BX = (B == 0), BY = (B == 1), BZ = (B == 2);
   return (at<X0 - 2 * BX, Y0 - 2 * BY, Z0 - 2 * BZ>(T) - at<X0 + 2 * BX, Y0 + 2 * BY, Z0 + 2 * BZ>(T)) +
          8.0 * (at<X0 + BX, Y0 + BY, Z0 + BZ>(T) - at<X0 - BX, Y0 - BY, Z0 - BZ>(T));
 }
-template <int A, int B>
-__device__ __forceinline__ double t_dd(const double *T, const Z4cGeo &G) {
+template <int A, int B, class L>
+__device__ __forceinline__ double t_dd(TP<L> T, const Z4cGeo &G) {
   return ((t_row<A, B, -2>(T) - t_row<A, B, 2>(T)) + 8.0 * (t_row<A, B, 1>(T) - t_row<A, B, -1>(T))) *
          G.i144hh[A + B - 1];
 }
 // lop-sided advection: pw = max(beta/h,0)/12, mw = min(beta/h,0)/12 per direction
-template <int A>
-__device__ __forceinline__ double t_adv1(const double *T, double pw, double mw) {
+template <int A, class L>
+__device__ __forceinline__ double t_adv1(TP<L> T, double pw, double mw) {
   return pw * ((18.0 * ax<A, 1>(T) - 3.0 * ax<A, -1>(T)) + (ax<A, 3>(T) - 6.0 * ax<A, 2>(T)) -
                10.0 * ax<A, 0>(T)) +
          mw * ((6.0 * ax<A, -2>(T) - ax<A, -3>(T)) + (3.0 * ax<A, 1>(T) - 18.0 * ax<A, -1>(T)) +
                10.0 * ax<A, 0>(T));
 }
-__device__ __forceinline__ double t_adv(const double *T, const double pw[3], const double mw[3]) {
+template <class L>
+__device__ __forceinline__ double t_adv(TP<L> T, const double pw[3], const double mw[3]) {
   return t_adv1<0>(T, pw[0], mw[0]) + t_adv1<1>(T, pw[1], mw[1]) + t_adv1<2>(T, pw[2], mw[2]);
 }
-template <int A>
-__device__ __forceinline__ double t_ko1(const double *T) {
+template <int A, class L>
+__device__ __forceinline__ double t_ko1(TP<L> T) {
   return (ax<A, -2>(T) + ax<A, 2>(T)) - 4.0 * (ax<A, -1>(T) + ax<A, 1>(T)) + 6.0 * ax<A, 0>(T);
 }
-__device__ __forceinline__ double t_ko(const double *T, const Z4cGeo &G) {
+template <class L>
+__device__ __forceinline__ double t_ko(TP<L> T, const Z4cGeo &G) {
   return -(t_ko1<0>(T) * G.ih[0] + t_ko1<1>(T) * G.ih[1] + t_ko1<2>(T) * G.ih[2]);
 }
 // all six second derivatives (xx xy xz yy yz zz)
-__device__ __forceinline__ void t_hess(const double *T, const Z4cGeo &G, double d[6]) {
+template <class L>
+__device__ __forceinline__ void t_hess(TP<L> T, const Z4cGeo &G, double d[6]) {
   d[0] = t_d2<0>(T, G);
   d[1] = t_dd<0, 1>(T, G);
   d[2] = t_dd<0, 2>(T, G);
@@ -624,31 +658,31 @@ struct Sten {
 
 // neighbour at a run-time offset o along A (|o| <= HALO); T is the thread's
 // tile corner, as for at<>
-template <int A>
-__device__ __forceinline__ double ts(const double *T, int o) {
-  return T[HALO * (1 + SX + SX * SY) + o * (A == 0 ? 1 : (A == 1 ? SX : SX * SY))];
+template <int A, class L>
+__device__ __forceinline__ double ts(TP<L> T, int o) {
+  return T.p[HALO * (1 + L::SX + L::SX * L::SY) + o * (A == 0 ? 1 : (A == 1 ? L::SX : L::SX * L::SY))];
 }
 
-template <bool SH, int A>
-__device__ __forceinline__ double s_d1(const double *T, const Z4cGeo &G, const Sten &S) {
+template <bool SH, int A, class L>
+__device__ __forceinline__ double s_d1(TP<L> T, const Z4cGeo &G, const Sten &S) {
   if (SH && S.o2) return (ax<A, 1>(T) - ax<A, -1>(T)) * G.i2h[A];
   return t_d1<A>(T, G);
 }
-template <bool SH>
-__device__ __forceinline__ void s_grad(const double *T, const Z4cGeo &G, const Sten &S, double d[3]) {
+template <bool SH, class L>
+__device__ __forceinline__ void s_grad(TP<L> T, const Z4cGeo &G, const Sten &S, double d[3]) {
   d[0] = s_d1<SH, 0>(T, G, S);
   d[1] = s_d1<SH, 1>(T, G, S);
   d[2] = s_d1<SH, 2>(T, G, S);
 }
-template <int A, int B>
-__device__ __forceinline__ double t_dd2(const double *T, const Z4cGeo &G) {
+template <int A, int B, class L>
+__device__ __forceinline__ double t_dd2(TP<L> T, const Z4cGeo &G) {
   constexpr int AX = (A == 0), AY = (A == 1), AZ = (A == 2), BX = (B == 0), BY = (B == 1), BZ = (B == 2);
   return ((at<AX + BX, AY + BY, AZ + BZ>(T) - at<AX - BX, AY - BY, AZ - BZ>(T)) -
           (at<-AX + BX, -AY + BY, -AZ + BZ>(T) - at<-AX - BX, -AY - BY, -AZ - BZ>(T))) *
          G.i4hh[A + B - 1];
 }
-template <bool SH>
-__device__ __forceinline__ void s_hess(const double *T, const Z4cGeo &G, const Sten &S, double d[6]) {
+template <bool SH, class L>
+__device__ __forceinline__ void s_hess(TP<L> T, const Z4cGeo &G, const Sten &S, double d[6]) {
   if (SH && S.o2) {
     d[0] = (ax<0, -1>(T) + ax<0, 1>(T) - 2.0 * ax<0, 0>(T)) * G.ihh[0];
     d[1] = t_dd2<0, 1>(T, G);
@@ -661,8 +695,8 @@ __device__ __forceinline__ void s_hess(const double *T, const Z4cGeo &G, const S
   t_hess(T, G, d);
 }
 // advection along A with the reference's row selection (Edge::set_adv)
-template <int A>
-__device__ __forceinline__ double s_adv1(const double *T, const Sten &S) {
+template <int A, class L>
+__device__ __forceinline__ double s_adv1(TP<L> T, const Sten &S) {
   const double wx = S.wx[A];
   if (wx == 0.0) return 0.0;
   const int sg = wx > 0.0 ? 1 : -1;
@@ -677,13 +711,13 @@ __device__ __forceinline__ double s_adv1(const double *T, const Sten &S) {
   }
   return -sg * wx * acc;
 }
-template <bool SH>
-__device__ __forceinline__ double s_adv(const double *T, const double pw[3], const double mw[3], const Sten &S) {
+template <bool SH, class L>
+__device__ __forceinline__ double s_adv(TP<L> T, const double pw[3], const double mw[3], const Sten &S) {
   if (SH && S.shift) return s_adv1<0>(T, S) + s_adv1<1>(T, S) + s_adv1<2>(T, S);
   return t_adv(T, pw, mw);
 }
-template <bool SH>
-__device__ __forceinline__ double s_ko(const double *T, const Z4cGeo &G, const Sten &S) {
+template <bool SH, class L>
+__device__ __forceinline__ double s_ko(TP<L> T, const Z4cGeo &G, const Sten &S) {
   if (SH && !S.ko) return 0.0;
   return t_ko(T, G);
 }
@@ -692,13 +726,14 @@ __device__ __forceinline__ double s_ko(const double *T, const Z4cGeo &G, const S
 // loads of the next field are issued into registers (NLOAD per thread, all
 // in flight together) before the current field is computed, and written to
 // LDS between two barriers at the start of the next stage.
-constexpr int NLOAD = (TILE + NTHR - 1) / NTHR;
 // staging order of the 25 fields: gt_ij, Gamma^i, chi, alpha, Khat, Theta,
 // At_ij, beta^i, B^i
 __device__ constexpr int STAGE_ORDER[25] = {0,  1,  2,  3,  4,  5,  14, 15, 16, 6,  18, 13, 17,
                                             7,  8,  9,  10, 11, 12, 19, 20, 21, 22, 23, 24};
 
+template <class L>
 struct Stager {
+  static constexpr int NLOAD = L::NLOAD;
   int goff[NLOAD];
   unsigned ok;      // bit k: slot k lies inside the tile and the box
   long cbase;       // linear index of the tile corner (i0-3, j0-3, k0-3); may lie before the box
@@ -709,9 +744,9 @@ struct Stager {
 #pragma unroll
     for (int k = 0; k < NLOAD; k++) {
       const int idx = tid + k * NTHR;
-      const int x = idx % SX, y = (idx / SX) % SY, z = idx / (SX * SY);
+      const int x = idx % L::SX, y = (idx / L::SX) % L::SY, z = idx / (L::SX * L::SY);
       const int gi = i0 - HALO + x, gj = j0 - HALO + y, gk = k0 - HALO + z;
-      const bool in = idx < TILE && gi >= 0 && gj >= 0 && gk >= 0 && gi < G.m && gj < G.n && gk < G.o;
+      const bool in = idx < L::TILE && gi >= 0 && gj >= 0 && gk >= 0 && gi < G.m && gj < G.n && gk < G.o;
       goff[k] = in ? (int)(x + G.sj * y + G.sk * z) : 0;
       ok |= in ? (1u << k) : 0u;
     }
@@ -725,10 +760,73 @@ struct Stager {
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < NLOAD; k++)
-      if (tid + k * NTHR < TILE) tile[tid + k * NTHR] = pf[k];
+      if (tid + k * NTHR < L::TILE) tile[tid + k * NTHR] = pf[k];
     __syncthreads();
   }
 };
+
+// Staging through the LDS-DMA path: global_load_lds_dword moves each field
+// straight from HBM into one of NB LDS buffers (no registers, no ds_write), so
+// NB - 1 fields are in flight while one is computed.  A wave's 64 lanes write
+// 256 contiguous bytes of LDS (lane l -> dword l), so the tile is cut into
+// 64-dword chunks, NPER per wave; dword d of the tile is the low (d even) or
+// high half of tile element d / 2.  Slots outside the box load the box's first
+// point (finite data that no computed point reads).
+#define BH_LDS(p) ((__attribute__((address_space(3))) void *)(p))
+template <class L, int NB>
+struct DmaStager {
+  static constexpr int NW = NTHR / 64;
+  static constexpr int NCH = (2 * L::TILE + 63) / 64;
+  static constexpr int NPER = (NCH + NW - 1) / NW;
+  static constexpr int BUFD = NPER * NW * 32;  // doubles per buffer, padded to whole chunks
+  static_assert(NB >= 2 && (NB - 2) * NPER <= 63, "vmcnt range");
+  unsigned boff[NPER];  // byte offset of each chunk's dword from the field's base
+  int wave;
+  __device__ DmaStager(const Z4cGeo &G, int i0, int j0, int k0, int tid) {
+    const int lane = tid & 63;
+    wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const long cbase = G.noffset + (i0 - HALO) + G.sj * (j0 - HALO) + G.sk * (k0 - HALO);
+#pragma unroll
+    for (int c = 0; c < NPER; c++) {
+      const int d = (wave * NPER + c) * 64 + lane;
+      const int e = d >> 1;
+      const int x = e % L::SX, y = (e / L::SX) % L::SY, z = e / (L::SX * L::SY);
+      const int gi = i0 - HALO + x, gj = j0 - HALO + y, gk = k0 - HALO + z;
+      const bool in = e < L::TILE && gi >= 0 && gj >= 0 && gk >= 0 && gi < G.m && gj < G.n && gk < G.o;
+      const long el = in ? cbase + x + G.sj * y + G.sk * z : G.noffset;
+      boff[c] = (unsigned)(el * 8 + (in ? (d & 1) * 4 : 0));
+    }
+  }
+  __device__ __forceinline__ void issue(const double *f, double *buf) const {
+#pragma unroll
+    for (int c = 0; c < NPER; c++)
+      __builtin_amdgcn_global_load_lds((const void *)((const char *)f + boff[c]),
+                                       BH_LDS(buf + (wave * NPER + c) * 32), 4, 0, 0);
+  }
+};
+// s_waitcnt vmcnt(n) lgkmcnt(0) (expcnt not waited), n a small constant
+template <int N>
+__device__ __forceinline__ void wait_vm_lgkm0() {
+  static_assert(N >= 0 && N <= 63, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4));
+}
+template <int NPER>
+__device__ __forceinline__ void wait_fields_after(int k) {
+  switch (k) {
+    case 0: wait_vm_lgkm0<0>(); break;
+    case 1: wait_vm_lgkm0<NPER>(); break;
+    case 2: wait_vm_lgkm0<(2 * NPER <= 63 ? 2 * NPER : 0)>(); break;
+    default: wait_vm_lgkm0<(3 * NPER <= 63 ? 3 * NPER : 0)>(); break;
+  }
+}
+// workgroup barrier without a memory fence (a fence would also wait for the
+// LDS-DMA loads still in flight); the empty asm statements keep the compiler
+// from moving memory accesses across it
+__device__ __forceinline__ void bare_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
 
 // Pin a value in registers at this point of the program: an empty volatile
 // asm that reads and writes it.  Placed after each staged field, it keeps the
@@ -742,23 +840,51 @@ __device__ __forceinline__ void pin(double (&a)[N]) {
   for (int i = 0; i < N; i++) pin(a[i]);
 }
 
-// SH = false: every point of [lo, hi] is >= 3 from each edge (standard
-// stencils).  SH = true: the box-edge shell; tiles cover [lo, hi], a block
-// whose tile holds no shell point returns at once, and only points closer
-// than 3 to an edge are computed, with the per-point rules of Sten.
-template <bool SH>
-__global__ void __launch_bounds__(NTHR) z4c_rhs_bulk(Z4cPtrs P, Z4cGeo G, Z4cK K, int *flag) {
-  __shared__ double tile[TILE];
+// Regions of one launch: index ranges [lo, hi] of boxes of a level (bulk
+// regions, or slabs of the box-edge shell), each with its box geometry, and
+// the first block of each region.  All boxes of a level are computed by one
+// launch per tile layout.
+constexpr int kMaxRegions = 8;
+struct TileJobs {
+  int n;
+  int blk0[kMaxRegions + 1];
+  Z4cGeo geo[kMaxRegions];
+};
+
+// SH = false: every point of the regions is >= 3 from each edge (standard
+// stencils).  SH = true: the regions are slabs of the box-edge shell; only
+// points closer than 3 to an edge are computed, with the per-point rules of
+// Sten.
+template <class L, int NB>
+struct StageBuf {  // LDS doubles of the staging buffers
+  static constexpr int n = NB * DmaStager<L, NB>::BUFD;
+};
+template <class L>
+struct StageBuf<L, 0> {
+  static constexpr int n = L::TILE;
+};
+
+// NB = 0: one LDS tile, staged through registers (Stager); NB >= 2: NB LDS
+// buffers filled by the LDS-DMA path (DmaStager).  OCC: blocks per CU the
+// register allocation must allow (2 caps a thread at 256 registers, with
+// spills, for two waves per SIMD).
+template <class L, bool SH, int NB, int OCC>
+__global__ void __launch_bounds__(NTHR, OCC) z4c_rhs_tile(Z4cPtrs P, TileJobs J, Z4cK K, int *flag) {
+  __shared__ double tile[StageBuf<L, NB>::n];
+  // region of this block (uniform): the regions' tiles are numbered in order
+  int r = 0;
+  while (r + 1 < J.n && (int)blockIdx.x >= J.blk0[r + 1]) r++;
+  const Z4cGeo G = J.geo[r];  // a copy: indexing the argument array by r would stage it in scratch
+  const int b = (int)blockIdx.x - J.blk0[r];
+  const int ntx = (G.hi[0] - G.lo[0] + L::TX) / L::TX, nty = (G.hi[1] - G.lo[1] + L::TY) / L::TY;
   const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
-  const int tid = tx + TX * (ty + TY * tz);
-  const int i0 = G.lo[0] + blockIdx.x * TX, j0 = G.lo[1] + blockIdx.y * TY, k0 = G.lo[2] + blockIdx.z * TZ;
+  const int tid = tx + L::TX * (ty + L::TY * tz);
+  const int i0 = G.lo[0] + (b % ntx) * L::TX, j0 = G.lo[1] + ((b / ntx) % nty) * L::TY,
+            k0 = G.lo[2] + (b / (ntx * nty)) * L::TZ;
   const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
   Sten S{};
   bool active = (i <= G.hi[0] && j <= G.hi[1] && k <= G.hi[2]);
   if (SH) {
-    // uniform per block: does the tile reach within 3 points of an edge?
-    const int ie = min(i0 + TX - 1, G.hi[0]), je = min(j0 + TY - 1, G.hi[1]), ke = min(k0 + TZ - 1, G.hi[2]);
-    if (i0 >= 3 && j0 >= 3 && k0 >= 3 && ie <= G.m - 4 && je <= G.n - 4 && ke <= G.o - 4) return;
     // positions past hi (padding threads, never stored) are clamped to hi, so
     // that their rows and stencil reach stay within the tables and the halo
     const int pos[3] = {min(i, G.hi[0]), min(j, G.hi[1]), min(k, G.hi[2])};
@@ -784,7 +910,8 @@ __global__ void __launch_bounds__(NTHR) z4c_rhs_bulk(Z4cPtrs P, Z4cGeo G, Z4cK K
     }
   }
   const long q = G.noffset + i + G.sj * j + G.sk * k;
-  const double *T = tile + tx + SX * (ty + SY * tz);
+  const int toff = tx + L::SX * (ty + L::SY * tz);
+  TP<L> T{tile + toff};
   const double *const *u = P.u;
 
   // point values; threads outside the box use flat data so that their
@@ -813,12 +940,31 @@ __global__ void __launch_bounds__(NTHR) z4c_rhs_bulk(Z4cPtrs P, Z4cGeo G, Z4cK K
     Theta = u[17][q];
     alpha = u[18][q];
   }
-  Stager st(G, i0, j0, k0, tid);
-  st.load(u[STAGE_ORDER[0]]);
-  // stage s: commit the prefetched field, then prefetch field s+1
+  using St = typename std::conditional<NB == 0, Stager<L>, DmaStager<L, (NB == 0 ? 2 : NB)>>::type;
+  St st(G, i0, j0, k0, tid);
+  constexpr int NBUF = NB == 0 ? 1 : NB;
+  constexpr int BUFD = StageBuf<L, NB>::n / NBUF;
+  if constexpr (NB == 0) {
+    st.load(u[STAGE_ORDER[0]]);
+  } else {
+#pragma unroll
+    for (int p = 0; p < NB - 1; p++) st.issue(u[STAGE_ORDER[p]], tile + p * BUFD);
+  }
+  // stage s: make field s visible in LDS, start the transfer of a later field
   auto advance = [&](int sidx) {
-    st.commit(tile, tid);
-    if (sidx + 1 < 25) st.load(u[STAGE_ORDER[sidx + 1]]);
+    if constexpr (NB == 0) {
+      st.commit(tile, tid);
+      if (sidx + 1 < 25) st.load(u[STAGE_ORDER[sidx + 1]]);
+    } else {
+      // this wave's loads of field sidx are done once no more than the later
+      // fields' loads are outstanding; the barrier makes every wave's done
+      wait_fields_after<St::NPER>(min(NB - 2, 24 - sidx));
+      bare_barrier();
+      // the buffer of field sidx + NB - 1 held field sidx - 1, whose reads
+      // every wave completed before the barrier
+      if (sidx + NB - 1 < 25) st.issue(u[STAGE_ORDER[sidx + NB - 1]], tile + ((sidx + NB - 1) % NB) * BUFD);
+      T.p = tile + (sidx % NB) * BUFD + toff;
+    }
   };
   double pw[3], mw[3];
 #pragma unroll
@@ -1223,14 +1369,54 @@ static void make_geo(const DevBox &B, Z4cGeo &G) {
   G.edge_only = 0;
 }
 
-// launch the RHS over one box; returns error code
-static int launch_box(const DevBox &B, const Z4cPtrs &P, const Z4cK &K, int *flag) {
+// tile regions of one level, by layout
+struct LevelTiles {
+  std::vector<Z4cGeo> bulk, sx, sy, sz;
+};
+
+static void add_region(std::vector<Z4cGeo> &v, const Z4cGeo &G0, const int lo[3], const int hi[3]) {
+  if (hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2]) return;
+  Z4cGeo G = G0;
+  for (int a = 0; a < 3; a++) {
+    G.lo[a] = lo[a];
+    G.hi[a] = hi[a];
+  }
+  v.push_back(G);
+}
+
+template <class L>
+static int ntiles(const Z4cGeo &G) {
+  return ((G.hi[0] - G.lo[0] + L::TX) / L::TX) * ((G.hi[1] - G.lo[1] + L::TY) / L::TY) *
+         ((G.hi[2] - G.lo[2] + L::TZ) / L::TZ);
+}
+
+// one launch per kMaxRegions regions of a layout
+template <class L, bool SH, int NB, int OCC>
+static int launch_regions(const Z4cPtrs &P, const std::vector<Z4cGeo> &regs, const Z4cK &K, int *flag) {
+  for (size_t r0 = 0; r0 < regs.size(); r0 += kMaxRegions) {
+    TileJobs J{};
+    J.n = (int)std::min<size_t>(kMaxRegions, regs.size() - r0);
+    int blk = 0;
+    for (int r = 0; r < J.n; r++) {
+      J.geo[r] = regs[r0 + r];
+      J.blk0[r] = blk;
+      blk += ntiles<L>(J.geo[r]);
+    }
+    J.blk0[J.n] = blk;
+    if (blk == 0) continue;
+    z4c_rhs_tile<L, SH, NB, OCC><<<(unsigned)blk, dim3(L::TX, L::TY, L::TZ), 0, stream()>>>(P, J, K, flag);
+    BH_HIP(hipGetLastError());
+  }
+  return BAMHIP_OK;
+}
+
+// Split one box into its bulk region (>= 3 points from every edge) and the
+// slabs of its edge shell (fast parameters), or launch the generic per-point
+// kernel for the other parameter choices.
+static int plan_box(const DevBox &B, const Z4cPtrs &P, const Z4cK &K, int *flag, bool fast, LevelTiles &T) {
   Z4cGeo G;
   make_geo(B, G);
   if (G.hi[0] < G.lo[0] || G.hi[1] < G.lo[1] || G.hi[2] < G.lo[2]) return BAMHIP_OK;
-  const bool fast = (K.order_centered == 4 && K.order_advection == 4 && K.lopsided == 1 &&
-                     (K.order_dissipation == 4 || K.order_dissipation == 0));
-  // bulk region: >= 3 points from every box edge
   int blo[3], bhi[3];
   const int dims[3] = {B.m, B.n, B.o};
   bool have_bulk = fast;
@@ -1239,29 +1425,31 @@ static int launch_box(const DevBox &B, const Z4cPtrs &P, const Z4cK &K, int *fla
     bhi[a] = G.hi[a] < dims[a] - 4 ? G.hi[a] : dims[a] - 4;
     if (bhi[a] < blo[a]) have_bulk = false;
   }
-  if (have_bulk) {
-    Z4cGeo Gb = G;
-    for (int a = 0; a < 3; a++) {
-      Gb.lo[a] = blo[a];
-      Gb.hi[a] = bhi[a];
-    }
-    dim3 grid((bhi[0] - blo[0] + TX) / TX, (bhi[1] - blo[1] + TY) / TY, (bhi[2] - blo[2] + TZ) / TZ);
-    {
-      TimerScope t("z4_rhs");
-      z4c_rhs_bulk<false><<<grid, dim3(TX, TY, TZ), 0, stream()>>>(P, Gb, K, flag);
-    }
-    BH_HIP(hipGetLastError());
-  }
+  if (have_bulk) add_region(T.bulk, G, blo, bhi);
   const bool need_edge = !have_bulk || blo[0] > G.lo[0] || bhi[0] < G.hi[0] || blo[1] > G.lo[1] ||
                          bhi[1] < G.hi[1] || blo[2] > G.lo[2] || bhi[2] < G.hi[2];
   if (need_edge && fast) {
-    // the shell of points closer than 3 to an edge: LDS tile kernel with the per-point rules
-    dim3 grid((G.hi[0] - G.lo[0] + TX) / TX, (G.hi[1] - G.lo[1] + TY) / TY, (G.hi[2] - G.lo[2] + TZ) / TZ);
-    {
-      TimerScope t("z4_rhs_edge");
-      z4c_rhs_bulk<true><<<grid, dim3(TX, TY, TZ), 0, stream()>>>(P, G, K, flag);
+    // The shell of points closer than 3 to an edge, as disjoint slabs: the
+    // layers along x over all y, z; along y over the x interior; along z over
+    // the x and y interior.  Layers of axis a: [lo, min(hi, 2)] and
+    // [max(dim - 3, end of the first + 1), hi].
+    int lay[3][2][2], ilo[3], ihi[3];
+    for (int a = 0; a < 3; a++) {
+      lay[a][0][0] = G.lo[a];
+      lay[a][0][1] = std::min(G.hi[a], 2);
+      lay[a][1][0] = std::max(dims[a] - 3, lay[a][0][1] + 1);
+      lay[a][1][1] = G.hi[a];
+      ilo[a] = std::max(G.lo[a], 3);
+      ihi[a] = std::min(G.hi[a], dims[a] - 4);
     }
-    BH_HIP(hipGetLastError());
+    for (int s = 0; s < 2; s++) {
+      const int xl[3] = {lay[0][s][0], G.lo[1], G.lo[2]}, xh[3] = {lay[0][s][1], G.hi[1], G.hi[2]};
+      add_region(T.sx, G, xl, xh);
+      const int yl[3] = {ilo[0], lay[1][s][0], G.lo[2]}, yh[3] = {ihi[0], lay[1][s][1], G.hi[2]};
+      add_region(T.sy, G, yl, yh);
+      const int zl[3] = {ilo[0], ilo[1], lay[2][s][0]}, zh[3] = {ihi[0], ihi[1], lay[2][s][1]};
+      add_region(T.sz, G, zl, zh);
+    }
   } else if (need_edge) {
     G.edge_only = have_bulk ? 1 : 0;
     dim3 grid((G.hi[0] - G.lo[0] + 64) / 64, (G.hi[1] - G.lo[1] + 4) / 4, G.hi[2] - G.lo[2] + 1);
@@ -1273,6 +1461,121 @@ static int launch_box(const DevBox &B, const Z4cPtrs &P, const Z4cK &K, int *fla
     BH_HIP(hipGetLastError());
   }
   return BAMHIP_OK;
+}
+
+// ---- autotuner of the tile kernels
+// A variant is one instantiation of the tile kernel: tile shape (TX x TY x
+// TZ, so the z depth of a block) and staging (registers, or LDS-DMA with NB
+// buffers).  Every variant evaluates the same per-point arithmetic in the same
+// order, so all give bitwise the same right-hand side.  The first time a set
+// of regions (a level's boxes) is seen outside graph capture, each variant is
+// timed on it (one warm-up launch, then two timed with HIP events) and the
+// fastest is kept for that set.
+using RegionLists = std::vector<const std::vector<Z4cGeo> *>;
+using LaunchFn = int (*)(const Z4cPtrs &, const RegionLists &, const Z4cK &, int *);
+struct Variant {
+  const char *name;
+  bool dma;
+  LaunchFn fn;
+};
+template <class L, int NB, int OCC>
+static int bulk_variant(const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, int *flag) {
+  return launch_regions<L, false, NB, OCC>(P, *R[0], K, flag);
+}
+template <int NB, int OCC>
+static int shell_variant(const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, int *flag) {
+  int e;
+  if ((e = launch_regions<LSlabX, true, NB, OCC>(P, *R[0], K, flag))) return e;
+  if ((e = launch_regions<LSlabY, true, NB, OCC>(P, *R[1], K, flag))) return e;
+  return launch_regions<LSlabZ, true, NB, OCC>(P, *R[2], K, flag);
+}
+constexpr int kMaxVariants = 8;
+static const Variant kVariants[2][kMaxVariants] = {
+    {{"bulk 16x4x4 dma3", true, bulk_variant<LBulk, 3, 1>},
+     {"bulk 16x4x4 reg", false, bulk_variant<LBulk, 0, 1>},
+     {"bulk 16x4x4 dma4", true, bulk_variant<LBulk, 4, 1>},
+     {"bulk 32x4x2 dma3", true, bulk_variant<Lay<32, 4, 2>, 3, 1>},
+     {"bulk 16x8x2 dma3", true, bulk_variant<Lay<16, 8, 2>, 3, 1>},
+     {"bulk 16x4x4 dma3 occ2", true, bulk_variant<LBulk, 3, 2>},
+     {"bulk 16x4x4 dma4 occ2", true, bulk_variant<LBulk, 4, 2>},
+     {"bulk 32x4x2 dma3 occ2", true, bulk_variant<Lay<32, 4, 2>, 3, 2>}},
+    {{"shell slabs dma3", true, shell_variant<3, 1>},
+     {"shell slabs reg", false, shell_variant<0, 1>},
+     {"shell slabs dma3 occ2", true, shell_variant<3, 2>}}};
+static const int kNumVariants[2] = {8, 3};
+// variant 0 of each kind is the default; variant 1 needs no LDS-DMA
+
+struct TuneEntry {
+  int choice;
+  double ms[kMaxVariants];
+};
+struct TuneState {
+  int mode = 1;  // 0: defaults only; 1: tune each new region set
+  int forced[2] = {-1, -1};
+  std::map<std::string, TuneEntry> chosen[2];
+};
+static TuneState g_tune;
+
+static std::string region_signature(const RegionLists &R) {
+  std::string s;
+  char buf[96];
+  for (const auto *v : R) {
+    s += '|';
+    for (const Z4cGeo &G : *v) {
+      snprintf(buf, sizeof buf, "%d,%d,%d:%d-%d,%d-%d,%d-%d;", G.m, G.n, G.o, G.lo[0], G.hi[0], G.lo[1], G.hi[1],
+               G.lo[2], G.hi[2]);
+      s += buf;
+    }
+  }
+  return s;
+}
+
+static int tuned_launch(int kind, const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, int *flag,
+                        bool repeatable, bool dma_ok) {
+  const Variant *V = kVariants[kind];
+  const int nv = kNumVariants[kind];
+  int idx = g_tune.forced[kind];
+  if (idx >= 0) {
+    if (V[idx].dma && !dma_ok)
+      return set_error(BAMHIP_EINVAL, "z4c tile variant '%s' needs a level below 2^29 points", V[idx].name);
+    return V[idx].fn(P, R, K, flag);
+  }
+  const std::string sig = region_signature(R);
+  auto it = g_tune.chosen[kind].find(sig);
+  if (it != g_tune.chosen[kind].end()) return V[it->second.choice].fn(P, R, K, flag);
+  const int dflt = dma_ok ? 0 : 1;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  BH_HIP(hipStreamIsCapturing(stream(), &cs));
+  if (g_tune.mode == 0 || !repeatable || cs != hipStreamCaptureStatusNone) return V[dflt].fn(P, R, K, flag);
+  TuneEntry te;
+  te.choice = dflt;
+  hipEvent_t a, b;
+  BH_HIP(hipEventCreate(&a));
+  BH_HIP(hipEventCreate(&b));
+  double best = 1e300;
+  int e = BAMHIP_OK;
+  for (int v = 0; v < nv && e == BAMHIP_OK; v++) {
+    te.ms[v] = -1.0;
+    if (V[v].dma && !dma_ok) continue;
+    if ((e = V[v].fn(P, R, K, flag))) break;
+    BH_HIP(hipEventRecord(a, stream()));
+    for (int rep = 0; rep < 2 && e == BAMHIP_OK; rep++) e = V[v].fn(P, R, K, flag);
+    BH_HIP(hipEventRecord(b, stream()));
+    BH_HIP(hipEventSynchronize(b));
+    float ms = 0.0f;
+    BH_HIP(hipEventElapsedTime(&ms, a, b));
+    te.ms[v] = ms / 2.0;
+    if (te.ms[v] < best) {
+      best = te.ms[v];
+      te.choice = v;
+    }
+  }
+  BH_HIP(hipEventDestroy(a));
+  BH_HIP(hipEventDestroy(b));
+  if (e) return e;
+  g_tune.chosen[kind][sig] = te;
+  // the outputs come from the chosen variant, as in every later call
+  return V[te.choice].fn(P, R, K, flag);
 }
 
 int z4c_rhs_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, const int *upre, double c,
@@ -1308,8 +1611,26 @@ int z4c_rhs_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int *une
       if (B.skip[f] < 1) return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: box face %d has no flagged layer", f);
   const Z4cK K = make_k(par, c);
   if (check_finite) set_int<<<1, 1, 0, stream()>>>(lev->d_flag, 0);
+  const bool fast = (K.order_centered == 4 && K.order_advection == 4 && K.lopsided == 1 &&
+                     (K.order_dissipation == 4 || K.order_dissipation == 0));
+  LevelTiles T;
   for (const DevBox &B : lev->boxes)
-    if ((e = launch_box(B, P, K, lev->d_flag))) return e;
+    if ((e = plan_box(B, P, K, lev->d_flag, fast, T))) return e;
+  // repeated launches (tuning) give the same result unless unew overwrites
+  // what the kernel reads
+  bool repeatable = true;
+  for (int v = 0; v < 25 && repeatable; v++)
+    for (int w = 0; w < 25; w++)
+      if (P.r[v] == P.p[w]) repeatable = false;
+  const bool dma_ok = (long)lev->npoints * 8 < (1L << 32);
+  if (!T.bulk.empty()) {
+    TimerScope t("z4_rhs");
+    if ((e = tuned_launch(0, P, {&T.bulk}, K, lev->d_flag, repeatable, dma_ok))) return e;
+  }
+  if (!T.sx.empty() || !T.sy.empty() || !T.sz.empty()) {
+    TimerScope t("z4_rhs_edge");
+    if ((e = tuned_launch(1, P, {&T.sx, &T.sy, &T.sz}, K, lev->d_flag, repeatable, dma_ok))) return e;
+  }
   if (check_finite) {
     int h = 0;
     BH_HIP(hipMemcpyAsync(&h, lev->d_flag, sizeof(int), hipMemcpyDeviceToHost, stream()));
@@ -1324,4 +1645,60 @@ int z4c_rhs_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int *une
 extern "C" int bamhip_z4c_rhs(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, const int *upre,
                               double c, const int *ucur) {
   return bamhip::z4c_rhs_impl(lev, par, unew, upre, c, ucur, lev ? lev->check_mode != 0 : true);
+}
+
+extern "C" int bamhip_z4c_tune_mode(int mode) {
+  if (mode != 0 && mode != 1) return bamhip::set_error(BAMHIP_EINVAL, "bamhip_z4c_tune_mode: mode %d (0, 1)", mode);
+  const int old = bamhip::g_tune.mode;
+  bamhip::g_tune.mode = mode;
+  return old;
+}
+
+extern "C" int bamhip_z4c_tune_clear(void) {
+  bamhip::g_tune.chosen[0].clear();
+  bamhip::g_tune.chosen[1].clear();
+  return BAMHIP_OK;
+}
+
+extern "C" int bamhip_z4c_variant_count(int kind) {
+  if (kind != 0 && kind != 1) return bamhip::set_error(BAMHIP_EINVAL, "bamhip_z4c_variant_count: kind %d", kind);
+  return bamhip::kNumVariants[kind];
+}
+
+extern "C" const char *bamhip_z4c_variant_name(int kind, int idx) {
+  if (kind != 0 && kind != 1) return nullptr;
+  if (idx < 0 || idx >= bamhip::kNumVariants[kind]) return nullptr;
+  return bamhip::kVariants[kind][idx].name;
+}
+
+extern "C" int bamhip_z4c_force_variant(int kind, int idx) {
+  if (kind != 0 && kind != 1) return bamhip::set_error(BAMHIP_EINVAL, "bamhip_z4c_force_variant: kind %d", kind);
+  if (idx < -1 || idx >= bamhip::kNumVariants[kind])
+    return bamhip::set_error(BAMHIP_EINVAL, "bamhip_z4c_force_variant: variant %d of %d", idx,
+                             bamhip::kNumVariants[kind]);
+  bamhip::g_tune.forced[kind] = idx;
+  return BAMHIP_OK;
+}
+
+extern "C" int bamhip_z4c_tune_report(char *buf, int len) {
+  std::string s;
+  char line[160];
+  for (int kind = 0; kind < 2; kind++)
+    for (const auto &kv : bamhip::g_tune.chosen[kind]) {
+      const bamhip::TuneEntry &te = kv.second;
+      snprintf(line, sizeof line, "%s <- %zu regions:", bamhip::kVariants[kind][te.choice].name,
+               (size_t)std::count(kv.first.begin(), kv.first.end(), ';'));
+      s += line;
+      for (int v = 0; v < bamhip::kNumVariants[kind]; v++) {
+        snprintf(line, sizeof line, " %s=%.4fms", bamhip::kVariants[kind][v].name, te.ms[v]);
+        s += line;
+      }
+      s += '\n';
+    }
+  if (buf && len > 0) {
+    const int n = (int)std::min<size_t>(s.size(), (size_t)len - 1);
+    memcpy(buf, s.data(), n);
+    buf[n] = '\0';
+  }
+  return (int)s.size() + 1;
 }

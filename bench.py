@@ -424,6 +424,8 @@ def main():
                            "workload": "192^3 periodic box (+3 ghosts), one RHS launch"}
         out["box192"] = {"value": box["value"], "unit": "grid-point-updates/s", "ms_per_step": box["ms_per_step"],
                          "breakdown_ms_per_step": box["breakdown_ms_per_step"], "ham_l2_after": box["ham_l2_after"]}
+    # tile kernel chosen by the autotuner for each tuned box set, with the time of every variant
+    out["rhs_tuning"] = B.tune_report().splitlines()
     if "c2p" in only:
         out["c2p"] = run_c2p(args)
     if not args.no_cpu:

@@ -119,6 +119,23 @@ typedef struct {
 int bamhip_z4c_rhs(bamhip_level *lev, const bamhip_z4c_params *par,
                    const int *unew, const int *upre, double c, const int *ucur);
 
+/* Autotuner of the RHS tile kernels (no counterpart in the reference, whose
+ * RHS is one OpenMP loop nest, z4_rhs_movpunc_N.c:23).  kind 0: the bulk
+ * kernel (points >= 3 from every box edge); kind 1: the box-edge shell.
+ * A variant fixes the tile shape (its z depth included) and the LDS staging;
+ * all variants give bitwise the same RHS.  In mode 1 (default) the first
+ * bamhip_z4c_rhs call on a new set of boxes, outside graph capture and with
+ * unew not aliasing upre, times every variant and keeps the fastest for that
+ * set; mode 0 uses variant 0.  bamhip_z4c_force_variant(kind, i) pins variant
+ * i (-1: back to the tuner).  tune_report writes one line per tuned set with
+ * the time of each variant and returns the length needed (with the NUL). */
+int bamhip_z4c_tune_mode(int mode);
+int bamhip_z4c_tune_clear(void);
+int bamhip_z4c_variant_count(int kind);
+const char *bamhip_z4c_variant_name(int kind, int idx);
+int bamhip_z4c_force_variant(int kind, int idx);
+int bamhip_z4c_tune_report(char *buf, int len);
+
 /* Replaces set_boundary(unew, upre, c, ucur) for the Z4c list
  * (src/utility/boundary/boundary.c:25-224): radiative condition with the far
  * limits and speeds of z4_startup (z4.c:200-228) on PHYBOUND faces, then the
@@ -162,6 +179,16 @@ int bamhip_z4c_algcon(bamhip_level *lev, const int *g, const int *A, int subtrac
  * ham: optional variable index receiving H (-1: not stored) */
 int bamhip_z4c_ham_l2(bamhip_level *lev, const int *u, double chipsipower, int ham,
                       double *l2);
+
+/* Hamiltonian and momentum constraints of computeadmconstraints
+ * (src/projects/adm/adm.c:14-74 -> adm_constraints_N.m:14-116, usepsi off,
+ * no matter; order 4, order 2 one point from the box edge) with the L2 norms
+ * of bampi_allreduce_norm2 (reduce.c:25-30,209-217) over inner points:
+ * l2[0] = |ham|, l2[1..3] = |mom_x|, |mom_y|, |mom_z|.  ham: optional variable
+ * index receiving H (-1: not stored); mom: optional 3 indices receiving
+ * mom_i (NULL: not stored). */
+int bamhip_z4c_constraints_l2(bamhip_level *lev, const int *u, double chipsipower, int ham,
+                              const int *mom, double *l2);
 
 /* --------------------------------------------------------------- con2prim */
 /* EOS and atmosphere snapshot read by the reference c2p (GRHD.ATM_*,

@@ -76,6 +76,8 @@ def lib():
                                       C.c_int, C.POINTER(DP), C.POINTER(DP), C.c_int]
         L.ob_z4c_ham_l2.restype = C.c_double
         L.ob_z4c_ham_l2.argtypes = [C.POINTER(Box), C.POINTER(DP), C.c_double, DP]
+        L.ob_z4c_constraints_l2.restype = None
+        L.ob_z4c_constraints_l2.argtypes = [C.POINTER(Box), C.POINTER(DP), C.c_double, DP, C.POINTER(DP), DP]
         L.ob_lagrange_coeffs.argtypes = [C.c_int, C.c_double, DP]
         L.ob_prolong_1d.argtypes = [C.c_int, C.c_int, DP, DP]
         L.ob_prolong_box.argtypes = [C.c_int, IP, IP, IP, DP, DP, C.c_int, C.c_int, C.c_int, C.c_int]
@@ -218,6 +220,18 @@ def z4c_ham_l2(box, u, chipsipower=-4.0, want_field=False):
     l2 = lib().ob_z4c_ham_l2(C.byref(box), _ptrs(u), chipsipower,
                              ham.ctypes.data_as(DP) if want_field else None)
     return (l2, ham) if want_field else l2
+
+
+def z4c_constraints_l2(box, u, chipsipower=-4.0, want_fields=False):
+    """L2 norms (ham, mom_x, mom_y, mom_z) over the inner points; with
+    want_fields also the fields ham (shape of a variable) and mom (3, ...)"""
+    l2 = np.zeros(4)
+    ham = np.zeros(u.shape[1:]) if want_fields else None
+    mom = np.zeros((3,) + u.shape[1:]) if want_fields else None
+    lib().ob_z4c_constraints_l2(C.byref(box), _ptrs(u), chipsipower,
+                                ham.ctypes.data_as(DP) if want_fields else None,
+                                _ptrs(mom) if want_fields else None, l2.ctypes.data_as(DP))
+    return (l2, ham, mom) if want_fields else l2
 
 
 def lagrange_coeffs(n, x):

@@ -99,6 +99,8 @@ void ob_vladd(long npts, double *r, double ca, const double *a, double cb, const
 /* Hamiltonian constraint of the Z4c data (via ADM, adm_constraints_N order 4)
  * at inner points; returns the L2 norm sqrt(sum H^2 / n) over them */
 double ob_z4c_ham_l2(const ob_box *b, const double *const *u, double chipsipower, double *ham);
+void ob_z4c_constraints_l2(const ob_box *b, const double *const *u, double chipsipower, double *ham,
+                           double *const *mom, double l2[4]);
 
 /* ---------------- interpolation (src/utility/interpolate) ----------------- */
 void ob_lagrange_coeffs(int n, double x, double *c);

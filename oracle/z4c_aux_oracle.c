@@ -306,10 +306,13 @@ int ob_z4c_rk4_step(const ob_box *b, const ob_z4c_params *p, double dt, int bc,
 }
 
 /* ---------------------------------------------------------------------------
- * Hamiltonian constraint (src/projects/adm/adm_constraints_N.m:14-108 with
- * usepsi off, no matter) from the ADM fields of z4_to_adm, 4th-order centred
- * (2nd order one point from the box edge); L2 norm of reduce.c:25-30. */
-double ob_z4c_ham_l2(const ob_box *b, const double *const *u, double chipsipower, double *ham)
+ * Hamiltonian and momentum constraints (src/projects/adm/adm_constraints_N.m:
+ * 14-116 with usepsi off, no matter) from the ADM fields of z4_to_adm,
+ * 4th-order centred (2nd order one point from the box edge); L2 norms of
+ * reduce.c:25-30 over the inner points.  l2[0] = |ham|, l2[1..3] = |mom_i|;
+ * ham and mom[0..2] receive the fields when not NULL. */
+void ob_z4c_constraints_l2(const ob_box *b, const double *const *u, double chipsipower, double *ham,
+                           double *const *mom, double l2[4])
 {
   const long np = (long)b->m * b->n * b->o;
   const int M = b->m, N = b->n, O = b->o;
@@ -322,16 +325,16 @@ double ob_z4c_ham_l2(const ob_box *b, const double *const *u, double chipsipower
     kadm[c] = buf + (6 + c) * np;
   }
   ob_z4c_to_adm(b, chipsipower, u, gadm, kadm);
-  double sum = 0;
+  double sum = 0, sm0 = 0, sm1 = 0, sm2 = 0;
   long cnt = 0;
-#pragma omp parallel for collapse(2) reduction(+ : sum, cnt) schedule(static)
+#pragma omp parallel for collapse(2) reduction(+ : sum, sm0, sm1, sm2, cnt) schedule(static)
   for (int k = b->skip[4]; k < O - b->skip[5]; k++)
     for (int j = b->skip[2]; j < N - b->skip[3]; j++)
       for (int i = b->skip[0]; i < M - b->skip[1]; i++) {
         const long q = i + st[1] * j + st[2] * k;
         const int away1 = (i == 1 || j == 1 || k == 1 || i == M - 2 || j == N - 2 || k == O - 2);
         const int ord = away1 ? 2 : 4;
-        double g[3][3], K[3][3], gi[3][3], dg[3][3][3], ddg[3][3][3][3];
+        double g[3][3], K[3][3], gi[3][3], dg[3][3][3], ddg[3][3][3][3], dK[3][3][3];
         for (int a = 0; a < 3; a++)
           for (int c = 0; c < 3; c++) {
             g[a][c] = gadm[SY[a][c]][q];
@@ -340,10 +343,12 @@ double ob_z4c_ham_l2(const ob_box *b, const double *const *u, double chipsipower
         for (int e = 0; e < 3; e++)
           for (int a = 0; a < 3; a++)
             for (int c = 0; c < 3; c++) {
-              const double *f = gadm[SY[a][c]];
+              const double *f = gadm[SY[a][c]], *kf = kadm[SY[a][c]];
               const long s = st[e];
               dg[e][a][c] = (ord == 2) ? (f[q + s] - f[q - s]) / (2 * h[e])
                           : (f[q - 2 * s] - 8 * f[q - s] + 8 * f[q + s] - f[q + 2 * s]) / (12 * h[e]);
+              dK[e][a][c] = (ord == 2) ? (kf[q + s] - kf[q - s]) / (2 * h[e])
+                          : (kf[q - 2 * s] - 8 * kf[q - s] + 8 * kf[q + s] - kf[q + 2 * s]) / (12 * h[e]);
               for (int e2 = 0; e2 < 3; e2++) {
                 const long s2 = st[e2];
                 double v;
@@ -399,10 +404,50 @@ double ob_z4c_ham_l2(const ob_box *b, const double *const *u, double chipsipower
           for (int bb = 0; bb < 3; bb++) KK += Kud[a][bb] * Kud[bb][a];
         }
         const double H = R + trK * trK - KK;
+        /* covariant derivative D_a K_bc and D^a K_bc */
+        double cdK[3][3][3], cdKu[3][3][3];
+        for (int a = 0; a < 3; a++)
+          for (int bb = 0; bb < 3; bb++)
+            for (int c = 0; c < 3; c++) {
+              double t = dK[a][bb][c];
+              for (int d = 0; d < 3; d++) t -= gam[d][a][bb] * K[d][c] + gam[d][a][c] * K[bb][d];
+              cdK[a][bb][c] = t;
+            }
+        for (int a = 0; a < 3; a++)
+          for (int bb = 0; bb < 3; bb++)
+            for (int c = 0; c < 3; c++) {
+              double t = 0;
+              for (int d = 0; d < 3; d++) t += gi[a][d] * cdK[d][bb][c];
+              cdKu[a][bb][c] = t;
+            }
+        /* mom^a = g^ab D^c K_bc - g^bc D^a K_bc; mom_a = g_ab mom^b */
+        double momu[3], M[3];
+        for (int a = 0; a < 3; a++) {
+          double t = 0;
+          for (int bb = 0; bb < 3; bb++)
+            for (int c = 0; c < 3; c++) t += gi[a][bb] * cdKu[c][bb][c] - gi[bb][c] * cdKu[a][bb][c];
+          momu[a] = t;
+        }
+        for (int a = 0; a < 3; a++) M[a] = g[a][0] * momu[0] + g[a][1] * momu[1] + g[a][2] * momu[2];
         if (ham) ham[q] = H;
+        if (mom)
+          for (int a = 0; a < 3; a++) mom[a][q] = M[a];
         sum += H * H;
+        sm0 += M[0] * M[0];
+        sm1 += M[1] * M[1];
+        sm2 += M[2] * M[2];
         cnt += 1;
       }
   free(buf);
-  return cnt ? sqrt(sum / cnt) : 0.0;
+  l2[0] = cnt ? sqrt(sum / cnt) : 0.0;
+  l2[1] = cnt ? sqrt(sm0 / cnt) : 0.0;
+  l2[2] = cnt ? sqrt(sm1 / cnt) : 0.0;
+  l2[3] = cnt ? sqrt(sm2 / cnt) : 0.0;
+}
+
+double ob_z4c_ham_l2(const ob_box *b, const double *const *u, double chipsipower, double *ham)
+{
+  double l2[4];
+  ob_z4c_constraints_l2(b, u, chipsipower, ham, NULL, l2);
+  return l2[0];
 }

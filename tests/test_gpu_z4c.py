@@ -102,6 +102,58 @@ def test_rhs_edge_points_bam_box():
     lev.free()
 
 
+def test_rhs_tile_variants_bitwise_equal_on_uneven_level():
+    """three uneven boxes in one level (one launch per tile layout covers
+    all of them): every tile variant of the bulk and of the box-edge shell
+    gives bitwise the same RHS, the tuner's choice included, and it matches
+    the oracle box by box"""
+    h = 0.2
+    shapes = [((20, 20, 20), (-1.9, -1.8, -2.0)), ((22, 17, 26), (2.3, -1.5, -2.4)), ((8, 9, 7), (-0.4, 2.6, 0.3))]
+    obs, us = [], []
+    for (m, n, o), x0 in shapes:
+        ob = O.make_box(m, n, o, *x0, h, skip=(1,) * 6)
+        X = grid(ob)
+        u = puncture_z4c(X, masses=((1.0, (0.05, -0.02, 0.03)),))
+        rng = np.random.default_rng(m * n)
+        u[19:22] = 0.1 * rng.standard_normal((3,) + X.shape[1:])
+        u[13] = 0.05 * np.sin(X[0]) * np.cos(X[1])
+        obs.append(ob)
+        us.append(u)
+    cpu = np.concatenate([O.z4c_rhs(ob, O.z4c_params(), u)[0].reshape(25, -1) for ob, u in zip(obs, us)], axis=1)
+    boxes = [B.make_box(ob.m, ob.n, ob.o, ob.x0, ob.y0, ob.z0, ob.dx, skip=(1,) * 6) for ob in obs]
+    lev = B.Level(boxes, 50)
+    lev.enable(range(50))
+    lev.put_state(U, np.concatenate([u.reshape(25, -1) for u in us], axis=1))
+    par = B.z4c_params()
+    results = {}
+    try:
+        for kind, other in ((B.BULK, B.SHELL), (B.SHELL, B.BULK)):
+            B.force_variant(other, 0)
+            for v, name in enumerate(B.variants(kind)):
+                B.force_variant(kind, v)
+                for r in R:
+                    lev.fill(r, 0.0)
+                lev.z4c_rhs(par, R, U)
+                results[name] = lev.get_state(R)
+            B.force_variant(kind, -1)
+            B.force_variant(other, -1)
+        B.tune_clear()
+        B.tune_mode(1)
+        lev.z4c_rhs(par, R, U)
+        results["tuned"] = lev.get_state(R)
+        report = B.tune_report()
+    finally:
+        B.force_variant(B.BULK, -1)
+        B.force_variant(B.SHELL, -1)
+    assert report.count("\n") == 2, report
+    first = next(iter(results.values()))
+    for name, got in results.items():
+        np.testing.assert_array_equal(got, first, err_msg=name)
+    err = _rel_err(first, cpu)
+    assert err.max() <= 1e-12, dict(zip(B.VAR_NAMES, err))
+    lev.free()
+
+
 @pytest.mark.parametrize("params", [dict(advection_lopsided=0), dict(order_centered=2, order_advection=2),
                                     dict(advection_lopsided=2, order_advection=4)])
 def test_rhs_generic_orders(params):
@@ -187,6 +239,35 @@ def test_rk4_puncture_box_radiative():
     assert err.max() <= 1e-11, dict(zip(B.VAR_NAMES, err))
 
 
+def test_constraint_norms_tracked_over_time():
+    """Hamiltonian and momentum constraint norms over 6 RK4 steps of the
+    small puncture box (adm.c:14-74 at every step), device against oracle:
+    the states agree to ~1e-11 of their change, so the norms agree to 1e-9"""
+    h = 0.5
+    n = 22
+    ob = O.make_box(n, n, n, -(n - 1) * h / 2, -(n - 1) * h / 2 + 0.1, -(n - 1) * h / 2 - 0.05, h, skip=(1,) * 6)
+    u = puncture_z4c(grid(ob), masses=((1.0, (0.05, -0.02, 0.03)),))
+    dt = 0.25 * h
+    lev = _level_for(ob, (B.PHYBOUND,) * 6)
+    lev.put_state(U, u.reshape(25, -1))
+    work = np.zeros((75,) + u.shape[1:])
+    par_o, par_g = O.z4c_params(), B.z4c_params()
+    history = []
+    for step in range(7):
+        lc = O.z4c_constraints_l2(ob, u)
+        lg = lev.z4c_constraints_l2(U)
+        history.append(lc)
+        np.testing.assert_allclose(lg, lc, rtol=1e-9, atol=0, err_msg="step %d" % step)
+        assert O.z4c_rk4_step(ob, par_o, dt, u, bc=1, pgh=0, work=work) == 0
+        lev.z4c_rk4_step(par_g, dt, U, P, R, V, pgh=0)
+    history = np.array(history)
+    assert np.all(np.isfinite(history)) and np.all(history[:, 0] > 0)
+    # K_ij = 0 initially, so the momentum constraint starts at exactly zero and
+    # is generated by the evolution
+    assert np.all(history[0, 1:] == 0) and np.all(history[1:, 1:] > 1e-4)
+    lev.free()
+
+
 def test_rk4_periodic_noise():
     """small version of the robust-stability run: flat space + 1e-10 noise,
     periodic with 3 ghost layers, 3 RK4 steps"""
@@ -236,6 +317,27 @@ def test_hamiltonian_norm_matches_oracle():
     lev.put_state(U, u.reshape(25, -1))
     l2g = lev.z4c_ham_l2(U)
     assert abs(l2g - l2c) <= 1e-11 * l2c
+    lev.free()
+
+
+def test_constraint_fields_and_norms_match_oracle():
+    """Hamiltonian and momentum constraint fields and their L2 norms
+    (adm.c:14-74) on perturbed Kerr-Schild data, including the order-2 points
+    one from the box edge"""
+    h = 0.1
+    ob = O.make_box(22, 24, 26, 1.8, 0.9, -1.7, h, skip=(1,) * 6)
+    u = _perturbed_ks(ob)
+    l2c, hc, mc = O.z4c_constraints_l2(ob, u, want_fields=True)
+    lev = _level_for(ob, (B.PHYBOUND,) * 6, nvars=29)
+    lev.put_state(U, u.reshape(25, -1))
+    l2g = lev.z4c_constraints_l2(U, ham=25, mom=[26, 27, 28])
+    np.testing.assert_allclose(l2g, l2c, rtol=1e-11, atol=0)
+    sl = (slice(1, -1),) * 3
+    for v, ref in zip(range(25, 29), [hc, *mc]):
+        got = lev.get(v).reshape(ref.shape)
+        scale = np.abs(ref[sl]).max()
+        assert scale > 1e-3
+        assert np.abs(got[sl] - ref[sl]).max() <= 1e-11 * scale, v
     lev.free()
 
 

@@ -59,6 +59,70 @@ def test_ks_stationary_nongauge_rhs_vanishes_at_4th_order(ks_runs):
     assert np.all(ratio > 11.0), ratio   # 16 for exact 4th order
 
 
+def test_ks_constraints_vanish_at_4th_order(ks_runs):
+    """Kerr-Schild data satisfy the Hamiltonian and momentum constraints; with
+    nonzero K_ij and Christoffel symbols a dropped connection term, a wrong
+    sign or a wrong contraction leaves an O(1) residual."""
+    _, runs = ks_runs
+    errs = []
+    for h in (0.08, 0.04):
+        b, X, u, r = runs[h]
+        l2, ham, mom = O.z4c_constraints_l2(b, u, want_fields=True)
+        e = [np.abs(ham[INNER]).max()] + [np.abs(mom[a][INNER]).max() for a in range(3)]
+        errs.append(np.array(e))
+        # the norms are over the same inner points (skip layers of 3)
+        assert np.allclose(l2, [np.sqrt(np.mean(f[INNER] ** 2)) for f in (ham, *mom)], rtol=1e-12)
+    assert errs[0].max() < 1e-4, errs[0]
+    ratio = errs[0] / errs[1]
+    assert np.all(ratio > 11.0), ratio   # 16 for exact 4th order
+
+
+def test_momentum_constraint_flat_polynomial_exact():
+    """flat metric with quadratic K_ij (At_ij and Khat): Christoffels vanish
+    and the finite differences are exact, so mom_i = d_j K_ij - d_i K exactly
+    at every inner point, including those one point from the edge (order 2)."""
+    h = 0.1
+    n = 12
+    b = O.make_box(n, n + 1, n + 2, -0.55, -0.6, -0.65, h, skip=(1,) * 6)
+    X = grid(b)
+    x, y, z = X
+    u = flat_z4c(x.shape)
+    SYM = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    rng = np.random.default_rng(7)
+    coef = rng.normal(size=(7, 10)) * 0.1
+
+    def quad(cf):
+        return (cf[0] + cf[1] * x + cf[2] * y + cf[3] * z + cf[4] * x * x + cf[5] * y * y + cf[6] * z * z
+                + cf[7] * x * y + cf[8] * x * z + cf[9] * y * z)
+
+    def dquad(cf):
+        return np.array([cf[1] + 2 * cf[4] * x + cf[7] * y + cf[8] * z,
+                         cf[2] + 2 * cf[5] * y + cf[7] * x + cf[9] * z,
+                         cf[3] + 2 * cf[6] * z + cf[8] * x + cf[9] * y])
+    for c in range(6):
+        u[7 + c] = quad(coef[c])
+    u[13] = quad(coef[6])
+    dK = np.zeros((3, 3, 3) + x.shape)     # dK[e, i, j] = d_e K_ij
+    dKh = dquad(coef[6])
+    for c, (i, j) in enumerate(SYM):
+        d = dquad(coef[c]) + (i == j) * dKh / 3.0
+        dK[:, i, j] = d
+        dK[:, j, i] = d
+    l2, ham, mom = O.z4c_constraints_l2(b, u, want_fields=True)
+    sl = (slice(1, -1),) * 3
+    for i in range(3):
+        expect = sum(dK[j, i, j] for j in range(3)) - sum(dK[i, j, j] for j in range(3))
+        assert np.abs(expect[sl]).max() > 1e-2
+        np.testing.assert_allclose(mom[i][sl], expect[sl], rtol=0, atol=1e-12)
+    # H = K^2 - K_ij K^ij for flat space
+    Kij = np.zeros((3, 3) + x.shape)
+    for c, (i, j) in enumerate(SYM):
+        Kij[i, j] = Kij[j, i] = u[7 + c] + (i == j) * u[13] / 3.0
+    trK = Kij[0, 0] + Kij[1, 1] + Kij[2, 2]
+    H = trK ** 2 - np.einsum("ij...,ij...->...", Kij, Kij)
+    np.testing.assert_allclose(ham[sl], H[sl], rtol=0, atol=1e-12)
+
+
 def test_ks_gauge_rhs_matches_closed_form(ks_runs):
     ks, runs = ks_runs
     errs = []
