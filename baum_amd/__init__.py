@@ -95,6 +95,7 @@ def lib():
     _sig(L, "bamhip_z4c_rhs", i, [vp, C.POINTER(Z4cParams), IP, IP, d, IP])
     _sig(L, "bamhip_z4c_tune_mode", i, [i])
     _sig(L, "bamhip_z4c_tune_clear", i, [])
+    _sig(L, "bamhip_z4c_set_overlap", i, [i])
     _sig(L, "bamhip_z4c_variant_count", i, [i])
     _sig(L, "bamhip_z4c_variant_name", C.c_char_p, [i, i])
     _sig(L, "bamhip_z4c_force_variant", i, [i, i])
@@ -620,6 +621,11 @@ def tune_mode(mode):
 
 def tune_clear():
     check(lib().bamhip_z4c_tune_clear(), "bamhip_z4c_tune_clear")
+
+
+def set_overlap(on=True):
+    """run the box-edge shell kernels concurrently with the bulk kernel"""
+    return lib().bamhip_z4c_set_overlap(1 if on else 0)
 
 
 def variants(kind):

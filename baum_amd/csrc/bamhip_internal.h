@@ -22,6 +22,9 @@ int check_hip(hipError_t e, const char *what);
   } while (0)
 
 hipStream_t stream();
+hipStream_t side_stream();
+int fork_side();
+int join_side();
 
 // device-side box description (one BAM box)
 struct DevBox {

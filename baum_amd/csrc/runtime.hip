@@ -26,6 +26,28 @@ int check_hip(hipError_t e, const char *what) {
 
 hipStream_t stream() { return g_stream; }
 
+// A second stream (default priority, like the library stream) for work that
+// may overlap the library stream's: fork makes it wait for everything issued
+// so far on the library stream, join makes the library stream wait for
+// everything issued on it since.  Both record events only, so a graph capture
+// of the library stream takes the side stream's work in as a parallel branch.
+static hipStream_t g_side = nullptr;
+static hipEvent_t g_fork = nullptr, g_join = nullptr;
+
+hipStream_t side_stream() { return g_side; }
+
+int fork_side() {
+  BH_HIP(hipEventRecord(g_fork, g_stream));
+  BH_HIP(hipStreamWaitEvent(g_side, g_fork, 0));
+  return BAMHIP_OK;
+}
+
+int join_side() {
+  BH_HIP(hipEventRecord(g_join, g_side));
+  BH_HIP(hipStreamWaitEvent(g_stream, g_join, 0));
+  return BAMHIP_OK;
+}
+
 int gather_ptrs(bamhip_level *lev, const int *idx, int n, double **out, const char *what) {
   if (!idx) return set_error(BAMHIP_EINVAL, "%s: variable list is NULL", what);
   for (int i = 0; i < n; i++) {
@@ -116,13 +138,27 @@ int bamhip_init(int device) {
   BH_HIP(hipSetDevice(device));
   if (!g_stream || g_device != device) {
     if (g_stream) (void)hipStreamDestroy(g_stream);
+    if (g_side) (void)hipStreamDestroy(g_side);
+    if (g_fork) (void)hipEventDestroy(g_fork);
+    if (g_join) (void)hipEventDestroy(g_join);
     BH_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    BH_HIP(hipStreamCreateWithFlags(&g_side, hipStreamNonBlocking));
+    BH_HIP(hipEventCreateWithFlags(&g_fork, hipEventDisableTiming));
+    BH_HIP(hipEventCreateWithFlags(&g_join, hipEventDisableTiming));
     g_device = device;
   }
   return BAMHIP_OK;
 }
 
 int bamhip_finalize(void) {
+  if (g_side) {
+    BH_HIP(hipStreamSynchronize(g_side));
+    BH_HIP(hipStreamDestroy(g_side));
+    g_side = nullptr;
+  }
+  if (g_fork) (void)hipEventDestroy(g_fork);
+  if (g_join) (void)hipEventDestroy(g_join);
+  g_fork = g_join = nullptr;
   if (g_stream) {
     BH_HIP(hipStreamSynchronize(g_stream));
     BH_HIP(hipStreamDestroy(g_stream));

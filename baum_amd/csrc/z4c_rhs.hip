@@ -573,6 +573,9 @@ using LBulk = Lay<16, 4, 4>;
 using LSlabX = Lay<2, 16, 8>;
 using LSlabY = Lay<16, 2, 8>;
 using LSlabZ = Lay<16, 8, 2>;
+using LLayX = Lay<1, 16, 16>;
+using LLayY = Lay<16, 1, 16>;
+using LLayZ = Lay<16, 16, 1>;
 
 // a thread's tile corner in the LDS tile of layout L (its point is at +HALO)
 template <class L>
@@ -844,12 +847,24 @@ __device__ __forceinline__ void pin(double (&a)[N]) {
 // regions, or slabs of the box-edge shell), each with its box geometry, and
 // the first block of each region.  All boxes of a level are computed by one
 // launch per tile layout.
-constexpr int kMaxRegions = 8;
+// The grid spacing constants are common to the regions of one launch (one
+// level), so a region carries only integer geometry.
+constexpr int kMaxRegions = 32;
+struct RegionGeo {
+  int m, n, o;
+  int lo[3], hi[3];
+  long noffset, sj, sk;
+};
+struct Spacing {
+  double h[3], ih[3], i12h[3], i12hh[3], i144hh[3], i2h[3], ihh[3], i4hh[3];
+};
 struct TileJobs {
   int n;
   int blk0[kMaxRegions + 1];
-  Z4cGeo geo[kMaxRegions];
+  RegionGeo reg[kMaxRegions];
+  Spacing sp;
 };
+static_assert(sizeof(TileJobs) <= 3072, "kernel argument size");
 
 // SH = false: every point of the regions is >= 3 from each edge (standard
 // stencils).  SH = true: the regions are slabs of the box-edge shell; only
@@ -874,7 +889,29 @@ __global__ void __launch_bounds__(NTHR, OCC) z4c_rhs_tile(Z4cPtrs P, TileJobs J,
   // region of this block (uniform): the regions' tiles are numbered in order
   int r = 0;
   while (r + 1 < J.n && (int)blockIdx.x >= J.blk0[r + 1]) r++;
-  const Z4cGeo G = J.geo[r];  // a copy: indexing the argument array by r would stage it in scratch
+  // copies: indexing the argument array by r through a reference would stage it in scratch
+  const RegionGeo Rg = J.reg[r];
+  Z4cGeo G;
+  G.m = Rg.m;
+  G.n = Rg.n;
+  G.o = Rg.o;
+  G.noffset = Rg.noffset;
+  G.sj = Rg.sj;
+  G.sk = Rg.sk;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    G.lo[a] = Rg.lo[a];
+    G.hi[a] = Rg.hi[a];
+    G.h[a] = J.sp.h[a];
+    G.ih[a] = J.sp.ih[a];
+    G.i12h[a] = J.sp.i12h[a];
+    G.i12hh[a] = J.sp.i12hh[a];
+    G.i144hh[a] = J.sp.i144hh[a];
+    G.i2h[a] = J.sp.i2h[a];
+    G.ihh[a] = J.sp.ihh[a];
+    G.i4hh[a] = J.sp.i4hh[a];
+  }
+  G.edge_only = 0;
   const int b = (int)blockIdx.x - J.blk0[r];
   const int ntx = (G.hi[0] - G.lo[0] + L::TX) / L::TX, nty = (G.hi[1] - G.lo[1] + L::TY) / L::TY;
   const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
@@ -1371,7 +1408,9 @@ static void make_geo(const DevBox &B, Z4cGeo &G) {
 
 // tile regions of one level, by layout
 struct LevelTiles {
-  std::vector<Z4cGeo> bulk, sx, sy, sz;
+  std::vector<Z4cGeo> bulk;
+  std::vector<Z4cGeo> sx, sy, sz;  // shell as slabs (up to two points thick) normal to x, y, z
+  std::vector<Z4cGeo> lx, ly, lz;  // the same slabs cut into single layers
 };
 
 static void add_region(std::vector<Z4cGeo> &v, const Z4cGeo &G0, const int lo[3], const int hi[3]) {
@@ -1391,20 +1430,49 @@ static int ntiles(const Z4cGeo &G) {
 }
 
 // one launch per kMaxRegions regions of a layout
+static bool same_spacing(const Z4cGeo &a, const Z4cGeo &b) {
+  return a.h[0] == b.h[0] && a.h[1] == b.h[1] && a.h[2] == b.h[2];
+}
+
+// one launch per kMaxRegions regions of a layout (and per grid spacing)
 template <class L, bool SH, int NB, int OCC>
-static int launch_regions(const Z4cPtrs &P, const std::vector<Z4cGeo> &regs, const Z4cK &K, int *flag) {
-  for (size_t r0 = 0; r0 < regs.size(); r0 += kMaxRegions) {
+static int launch_regions(const Z4cPtrs &P, const std::vector<Z4cGeo> &regs, const Z4cK &K, int *flag,
+                          hipStream_t st) {
+  size_t r0 = 0;
+  while (r0 < regs.size()) {
     TileJobs J{};
-    J.n = (int)std::min<size_t>(kMaxRegions, regs.size() - r0);
+    const Z4cGeo &G0 = regs[r0];
+    for (int a = 0; a < 3; a++) {
+      J.sp.h[a] = G0.h[a];
+      J.sp.ih[a] = G0.ih[a];
+      J.sp.i12h[a] = G0.i12h[a];
+      J.sp.i12hh[a] = G0.i12hh[a];
+      J.sp.i144hh[a] = G0.i144hh[a];
+      J.sp.i2h[a] = G0.i2h[a];
+      J.sp.ihh[a] = G0.ihh[a];
+      J.sp.i4hh[a] = G0.i4hh[a];
+    }
     int blk = 0;
-    for (int r = 0; r < J.n; r++) {
-      J.geo[r] = regs[r0 + r];
-      J.blk0[r] = blk;
-      blk += ntiles<L>(J.geo[r]);
+    J.n = 0;
+    while (r0 < regs.size() && J.n < kMaxRegions && same_spacing(regs[r0], G0)) {
+      const Z4cGeo &G = regs[r0++];
+      RegionGeo &Rg = J.reg[J.n];
+      Rg.m = G.m;
+      Rg.n = G.n;
+      Rg.o = G.o;
+      Rg.noffset = G.noffset;
+      Rg.sj = G.sj;
+      Rg.sk = G.sk;
+      for (int a = 0; a < 3; a++) {
+        Rg.lo[a] = G.lo[a];
+        Rg.hi[a] = G.hi[a];
+      }
+      J.blk0[J.n++] = blk;
+      blk += ntiles<L>(G);
     }
     J.blk0[J.n] = blk;
     if (blk == 0) continue;
-    z4c_rhs_tile<L, SH, NB, OCC><<<(unsigned)blk, dim3(L::TX, L::TY, L::TZ), 0, stream()>>>(P, J, K, flag);
+    z4c_rhs_tile<L, SH, NB, OCC><<<(unsigned)blk, dim3(L::TX, L::TY, L::TZ), 0, st>>>(P, J, K, flag);
     BH_HIP(hipGetLastError());
   }
   return BAMHIP_OK;
@@ -1449,6 +1517,16 @@ static int plan_box(const DevBox &B, const Z4cPtrs &P, const Z4cK &K, int *flag,
       add_region(T.sy, G, yl, yh);
       const int zl[3] = {ilo[0], ilo[1], lay[2][s][0]}, zh[3] = {ihi[0], ihi[1], lay[2][s][1]};
       add_region(T.sz, G, zl, zh);
+      // single layers: the distance to the edge along the slab normal is then
+      // uniform over a layer, and so are the stencil rules for most points
+      const int *lo3[3] = {xl, yl, zl}, *hi3[3] = {xh, yh, zh};
+      std::vector<Z4cGeo> *dst[3] = {&T.lx, &T.ly, &T.lz};
+      for (int a = 0; a < 3; a++)
+        for (int c = lo3[a][a]; c <= hi3[a][a]; c++) {
+          int l1[3] = {lo3[a][0], lo3[a][1], lo3[a][2]}, h1[3] = {hi3[a][0], hi3[a][1], hi3[a][2]};
+          l1[a] = h1[a] = c;
+          add_region(*dst[a], G, l1, h1);
+        }
     }
   } else if (need_edge) {
     G.edge_only = have_bulk ? 1 : 0;
@@ -1472,22 +1550,29 @@ static int plan_box(const DevBox &B, const Z4cPtrs &P, const Z4cK &K, int *flag,
 // timed on it (one warm-up launch, then two timed with HIP events) and the
 // fastest is kept for that set.
 using RegionLists = std::vector<const std::vector<Z4cGeo> *>;
-using LaunchFn = int (*)(const Z4cPtrs &, const RegionLists &, const Z4cK &, int *);
+using LaunchFn = int (*)(const Z4cPtrs &, const RegionLists &, const Z4cK &, int *, hipStream_t);
 struct Variant {
   const char *name;
   bool dma;
   LaunchFn fn;
 };
 template <class L, int NB, int OCC>
-static int bulk_variant(const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, int *flag) {
-  return launch_regions<L, false, NB, OCC>(P, *R[0], K, flag);
+static int bulk_variant(const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, int *flag, hipStream_t st) {
+  return launch_regions<L, false, NB, OCC>(P, *R[0], K, flag, st);
 }
 template <int NB, int OCC>
-static int shell_variant(const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, int *flag) {
+static int shell_variant(const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, int *flag, hipStream_t st) {
   int e;
-  if ((e = launch_regions<LSlabX, true, NB, OCC>(P, *R[0], K, flag))) return e;
-  if ((e = launch_regions<LSlabY, true, NB, OCC>(P, *R[1], K, flag))) return e;
-  return launch_regions<LSlabZ, true, NB, OCC>(P, *R[2], K, flag);
+  if ((e = launch_regions<LSlabX, true, NB, OCC>(P, *R[0], K, flag, st))) return e;
+  if ((e = launch_regions<LSlabY, true, NB, OCC>(P, *R[1], K, flag, st))) return e;
+  return launch_regions<LSlabZ, true, NB, OCC>(P, *R[2], K, flag, st);
+}
+template <int NB, int OCC>
+static int layer_variant(const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, int *flag, hipStream_t st) {
+  int e;
+  if ((e = launch_regions<LLayX, true, NB, OCC>(P, *R[3], K, flag, st))) return e;
+  if ((e = launch_regions<LLayY, true, NB, OCC>(P, *R[4], K, flag, st))) return e;
+  return launch_regions<LLayZ, true, NB, OCC>(P, *R[5], K, flag, st);
 }
 constexpr int kMaxVariants = 8;
 static const Variant kVariants[2][kMaxVariants] = {
@@ -1499,10 +1584,11 @@ static const Variant kVariants[2][kMaxVariants] = {
      {"bulk 16x4x4 dma3 occ2", true, bulk_variant<LBulk, 3, 2>},
      {"bulk 16x4x4 dma4 occ2", true, bulk_variant<LBulk, 4, 2>},
      {"bulk 32x4x2 dma3 occ2", true, bulk_variant<Lay<32, 4, 2>, 3, 2>}},
-    {{"shell slabs dma3", true, shell_variant<3, 1>},
+    {{"shell layers dma3", true, layer_variant<3, 1>},
      {"shell slabs reg", false, shell_variant<0, 1>},
-     {"shell slabs dma3 occ2", true, shell_variant<3, 2>}}};
-static const int kNumVariants[2] = {8, 3};
+     {"shell slabs dma3", true, shell_variant<3, 1>},
+     {"shell layers dma4", true, layer_variant<4, 1>}}};
+static const int kNumVariants[2] = {8, 4};
 // variant 0 of each kind is the default; variant 1 needs no LDS-DMA
 
 struct TuneEntry {
@@ -1510,6 +1596,7 @@ struct TuneEntry {
   double ms[kMaxVariants];
 };
 struct TuneState {
+  int overlap = 1;  // shell on the side stream, concurrent with the bulk
   int mode = 1;  // 0: defaults only; 1: tune each new region set
   int forced[2] = {-1, -1};
   std::map<std::string, TuneEntry> chosen[2];
@@ -1530,23 +1617,33 @@ static std::string region_signature(const RegionLists &R) {
   return s;
 }
 
-static int tuned_launch(int kind, const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, int *flag,
-                        bool repeatable, bool dma_ok) {
+// Variant of kind for the region lists R: the pinned one, the one kept for
+// this region set, or (mode 1, outside capture, repeatable launches) the
+// fastest after timing all of them on the library stream.  In the last case
+// the chosen variant has also been run once more, so the outputs are in place
+// and *done is set.
+static int choose_variant(int kind, const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, int *flag,
+                          bool repeatable, bool dma_ok, int *idx, bool *done) {
   const Variant *V = kVariants[kind];
   const int nv = kNumVariants[kind];
-  int idx = g_tune.forced[kind];
-  if (idx >= 0) {
-    if (V[idx].dma && !dma_ok)
-      return set_error(BAMHIP_EINVAL, "z4c tile variant '%s' needs a level below 2^29 points", V[idx].name);
-    return V[idx].fn(P, R, K, flag);
+  *done = false;
+  *idx = g_tune.forced[kind];
+  if (*idx >= 0) {
+    if (V[*idx].dma && !dma_ok)
+      return set_error(BAMHIP_EINVAL, "z4c tile variant '%s' needs a level below 2^29 points", V[*idx].name);
+    return BAMHIP_OK;
   }
   const std::string sig = region_signature(R);
   auto it = g_tune.chosen[kind].find(sig);
-  if (it != g_tune.chosen[kind].end()) return V[it->second.choice].fn(P, R, K, flag);
+  if (it != g_tune.chosen[kind].end()) {
+    *idx = it->second.choice;
+    return BAMHIP_OK;
+  }
   const int dflt = dma_ok ? 0 : 1;
+  *idx = dflt;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   BH_HIP(hipStreamIsCapturing(stream(), &cs));
-  if (g_tune.mode == 0 || !repeatable || cs != hipStreamCaptureStatusNone) return V[dflt].fn(P, R, K, flag);
+  if (g_tune.mode == 0 || !repeatable || cs != hipStreamCaptureStatusNone) return BAMHIP_OK;
   TuneEntry te;
   te.choice = dflt;
   hipEvent_t a, b;
@@ -1557,9 +1654,9 @@ static int tuned_launch(int kind, const Z4cPtrs &P, const RegionLists &R, const 
   for (int v = 0; v < nv && e == BAMHIP_OK; v++) {
     te.ms[v] = -1.0;
     if (V[v].dma && !dma_ok) continue;
-    if ((e = V[v].fn(P, R, K, flag))) break;
+    if ((e = V[v].fn(P, R, K, flag, stream()))) break;
     BH_HIP(hipEventRecord(a, stream()));
-    for (int rep = 0; rep < 2 && e == BAMHIP_OK; rep++) e = V[v].fn(P, R, K, flag);
+    for (int rep = 0; rep < 2 && e == BAMHIP_OK; rep++) e = V[v].fn(P, R, K, flag, stream());
     BH_HIP(hipEventRecord(b, stream()));
     BH_HIP(hipEventSynchronize(b));
     float ms = 0.0f;
@@ -1574,8 +1671,10 @@ static int tuned_launch(int kind, const Z4cPtrs &P, const RegionLists &R, const 
   BH_HIP(hipEventDestroy(b));
   if (e) return e;
   g_tune.chosen[kind][sig] = te;
+  *idx = te.choice;
   // the outputs come from the chosen variant, as in every later call
-  return V[te.choice].fn(P, R, K, flag);
+  *done = true;
+  return V[te.choice].fn(P, R, K, flag, stream());
 }
 
 int z4c_rhs_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, const int *upre, double c,
@@ -1623,13 +1722,30 @@ int z4c_rhs_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int *une
     for (int w = 0; w < 25; w++)
       if (P.r[v] == P.p[w]) repeatable = false;
   const bool dma_ok = (long)lev->npoints * 8 < (1L << 32);
-  if (!T.bulk.empty()) {
+  const RegionLists RB = {&T.bulk}, RS = {&T.sx, &T.sy, &T.sz, &T.lx, &T.ly, &T.lz};
+  const bool have_shell = !T.sx.empty() || !T.sy.empty() || !T.sz.empty();
+  int ib = -1, is = -1;
+  bool db = false, ds = false;
+  if (!T.bulk.empty() && (e = choose_variant(0, P, RB, K, lev->d_flag, repeatable, dma_ok, &ib, &db))) return e;
+  if (have_shell && (e = choose_variant(1, P, RS, K, lev->d_flag, repeatable, dma_ok, &is, &ds))) return e;
+  if (ib >= 0 && is >= 0 && !db && !ds && g_tune.overlap) {
+    // the shell on the side stream beside the bulk: the two write disjoint
+    // points, and the shell's few blocks fill the CUs the bulk's last wave of
+    // blocks leaves idle
     TimerScope t("z4_rhs");
-    if ((e = tuned_launch(0, P, {&T.bulk}, K, lev->d_flag, repeatable, dma_ok))) return e;
-  }
-  if (!T.sx.empty() || !T.sy.empty() || !T.sz.empty()) {
-    TimerScope t("z4_rhs_edge");
-    if ((e = tuned_launch(1, P, {&T.sx, &T.sy, &T.sz}, K, lev->d_flag, repeatable, dma_ok))) return e;
+    if ((e = fork_side())) return e;
+    if ((e = kVariants[1][is].fn(P, RS, K, lev->d_flag, side_stream()))) return e;
+    if ((e = kVariants[0][ib].fn(P, RB, K, lev->d_flag, stream()))) return e;
+    if ((e = join_side())) return e;
+  } else {
+    if (ib >= 0 && !db) {
+      TimerScope t("z4_rhs");
+      if ((e = kVariants[0][ib].fn(P, RB, K, lev->d_flag, stream()))) return e;
+    }
+    if (is >= 0 && !ds) {
+      TimerScope t("z4_rhs_edge");
+      if ((e = kVariants[1][is].fn(P, RS, K, lev->d_flag, stream()))) return e;
+    }
   }
   if (check_finite) {
     int h = 0;
@@ -1651,6 +1767,12 @@ extern "C" int bamhip_z4c_tune_mode(int mode) {
   if (mode != 0 && mode != 1) return bamhip::set_error(BAMHIP_EINVAL, "bamhip_z4c_tune_mode: mode %d (0, 1)", mode);
   const int old = bamhip::g_tune.mode;
   bamhip::g_tune.mode = mode;
+  return old;
+}
+
+extern "C" int bamhip_z4c_set_overlap(int on) {
+  const int old = bamhip::g_tune.overlap;
+  bamhip::g_tune.overlap = on != 0;
   return old;
 }
 

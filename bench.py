@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--c2p-n", type=int, default=1 << 24)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rhs-overlap", type=int, default=1,
+                    help="1: box-edge shell kernels concurrent with the bulk kernel (second stream)")
     ap.add_argument("--only", default="fmr,box192,c2p")
     return ap.parse_args()
 
@@ -398,6 +400,7 @@ def main():
     only = set(args.only.split(","))
     import baum_amd as B
     B.init()
+    B.set_overlap(args.rhs_overlap != 0)
     out = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
     fmr = run_fmr(args) if "fmr" in only else None

@@ -130,6 +130,9 @@ int bamhip_z4c_rhs(bamhip_level *lev, const bamhip_z4c_params *par,
  * i (-1: back to the tuner).  tune_report writes one line per tuned set with
  * the time of each variant and returns the length needed (with the NUL). */
 int bamhip_z4c_tune_mode(int mode);
+/* on != 0 (default): the shell kernels run on a second stream concurrently
+ * with the bulk kernel (both in a captured graph too); returns the old value */
+int bamhip_z4c_set_overlap(int on);
 int bamhip_z4c_tune_clear(void);
 int bamhip_z4c_variant_count(int kind);
 const char *bamhip_z4c_variant_name(int kind, int idx);
