@@ -768,22 +768,29 @@ struct Stager {
   }
 };
 
-// Staging through the LDS-DMA path: global_load_lds_dword moves each field
+// Staging through the LDS-DMA path: global_load_lds moves each field
 // straight from HBM into one of NB LDS buffers (no registers, no ds_write), so
 // NB - 1 fields are in flight while one is computed.  A wave's 64 lanes write
-// 256 contiguous bytes of LDS (lane l -> dword l), so the tile is cut into
-// 64-dword chunks, NPER per wave; dword d of the tile is the low (d even) or
-// high half of tile element d / 2.  Slots outside the box load the box's first
-// point (finite data that no computed point reads).
+// 64 consecutive units of LDS (lane l -> unit l), so the tile is cut into
+// chunks of 64 units, NPER per wave.  VEC = 1: a unit is one dword, the low
+// (even unit) or high half of a tile element; VEC = 4: a unit is 16 bytes,
+// the two tile elements (x, x + 1) with x even, which needs an even row
+// length and 16-byte aligned pairs in HBM (checked by the host).  Slots
+// outside the box load the box's first point (finite data that no computed
+// point reads).
 #define BH_LDS(p) ((__attribute__((address_space(3))) void *)(p))
-template <class L, int NB>
+template <class L, int NB, int VEC>
 struct DmaStager {
+  static_assert(VEC == 1 || VEC == 4, "dword or 16-byte units");
+  static_assert(VEC == 1 || (L::SX % 2 == 0 && L::TX % 2 == 0), "16-byte units need even rows and tile steps");
   static constexpr int NW = NTHR / 64;
-  static constexpr int NCH = (2 * L::TILE + 63) / 64;
+  static constexpr int NUNIT = VEC == 1 ? 2 * L::TILE : L::TILE / 2;
+  static constexpr int NCH = (NUNIT + 63) / 64;
   static constexpr int NPER = (NCH + NW - 1) / NW;
-  static constexpr int BUFD = NPER * NW * 32;  // doubles per buffer, padded to whole chunks
+  static constexpr int CHD = 64 * VEC / 2;      // doubles per chunk
+  static constexpr int BUFD = NPER * NW * CHD;  // doubles per buffer, padded to whole chunks
   static_assert(NB >= 2 && (NB - 2) * NPER <= 63, "vmcnt range");
-  unsigned boff[NPER];  // byte offset of each chunk's dword from the field's base
+  unsigned boff[NPER];  // byte offset of each chunk's unit from the field's base
   int wave;
   __device__ DmaStager(const Z4cGeo &G, int i0, int j0, int k0, int tid) {
     const int lane = tid & 63;
@@ -792,19 +799,23 @@ struct DmaStager {
 #pragma unroll
     for (int c = 0; c < NPER; c++) {
       const int d = (wave * NPER + c) * 64 + lane;
-      const int e = d >> 1;
+      const int e = VEC == 1 ? d >> 1 : 2 * d;
       const int x = e % L::SX, y = (e / L::SX) % L::SY, z = e / (L::SX * L::SY);
       const int gi = i0 - HALO + x, gj = j0 - HALO + y, gk = k0 - HALO + z;
       const bool in = e < L::TILE && gi >= 0 && gj >= 0 && gk >= 0 && gi < G.m && gj < G.n && gk < G.o;
       const long el = in ? cbase + x + G.sj * y + G.sk * z : G.noffset;
-      boff[c] = (unsigned)(el * 8 + (in ? (d & 1) * 4 : 0));
+      boff[c] = (unsigned)(el * 8 + (VEC == 1 && in ? (d & 1) * 4 : 0));
     }
   }
   __device__ __forceinline__ void issue(const double *f, double *buf) const {
 #pragma unroll
-    for (int c = 0; c < NPER; c++)
-      __builtin_amdgcn_global_load_lds((const void *)((const char *)f + boff[c]),
-                                       BH_LDS(buf + (wave * NPER + c) * 32), 4, 0, 0);
+    for (int c = 0; c < NPER; c++) {
+      const void *src = (const void *)((const char *)f + boff[c]);
+      if constexpr (VEC == 1)
+        __builtin_amdgcn_global_load_lds(src, BH_LDS(buf + (wave * NPER + c) * CHD), 4, 0, 0);
+      else
+        __builtin_amdgcn_global_load_lds(src, BH_LDS(buf + (wave * NPER + c) * CHD), 16, 0, 0);
+    }
   }
 };
 // s_waitcnt vmcnt(n) lgkmcnt(0) (expcnt not waited), n a small constant
@@ -870,12 +881,12 @@ static_assert(sizeof(TileJobs) <= 3072, "kernel argument size");
 // stencils).  SH = true: the regions are slabs of the box-edge shell; only
 // points closer than 3 to an edge are computed, with the per-point rules of
 // Sten.
-template <class L, int NB>
+template <class L, int NB, int VEC>
 struct StageBuf {  // LDS doubles of the staging buffers
-  static constexpr int n = NB * DmaStager<L, NB>::BUFD;
+  static constexpr int n = NB * DmaStager<L, NB, VEC>::BUFD;
 };
-template <class L>
-struct StageBuf<L, 0> {
+template <class L, int VEC>
+struct StageBuf<L, 0, VEC> {
   static constexpr int n = L::TILE;
 };
 
@@ -883,9 +894,9 @@ struct StageBuf<L, 0> {
 // buffers filled by the LDS-DMA path (DmaStager).  OCC: blocks per CU the
 // register allocation must allow (2 caps a thread at 256 registers, with
 // spills, for two waves per SIMD).
-template <class L, bool SH, int NB, int OCC>
+template <class L, bool SH, int NB, int VEC, int OCC>
 __global__ void __launch_bounds__(NTHR, OCC) z4c_rhs_tile(Z4cPtrs P, TileJobs J, Z4cK K, int *flag) {
-  __shared__ double tile[StageBuf<L, NB>::n];
+  __shared__ double tile[StageBuf<L, NB, VEC>::n];
   // region of this block (uniform): the regions' tiles are numbered in order
   int r = 0;
   while (r + 1 < J.n && (int)blockIdx.x >= J.blk0[r + 1]) r++;
@@ -977,10 +988,10 @@ __global__ void __launch_bounds__(NTHR, OCC) z4c_rhs_tile(Z4cPtrs P, TileJobs J,
     Theta = u[17][q];
     alpha = u[18][q];
   }
-  using St = typename std::conditional<NB == 0, Stager<L>, DmaStager<L, (NB == 0 ? 2 : NB)>>::type;
+  using St = typename std::conditional<NB == 0, Stager<L>, DmaStager<L, (NB == 0 ? 2 : NB), VEC>>::type;
   St st(G, i0, j0, k0, tid);
   constexpr int NBUF = NB == 0 ? 1 : NB;
-  constexpr int BUFD = StageBuf<L, NB>::n / NBUF;
+  constexpr int BUFD = StageBuf<L, NB, VEC>::n / NBUF;
   if constexpr (NB == 0) {
     st.load(u[STAGE_ORDER[0]]);
   } else {
@@ -1435,7 +1446,7 @@ static bool same_spacing(const Z4cGeo &a, const Z4cGeo &b) {
 }
 
 // one launch per kMaxRegions regions of a layout (and per grid spacing)
-template <class L, bool SH, int NB, int OCC>
+template <class L, bool SH, int NB, int VEC, int OCC>
 static int launch_regions(const Z4cPtrs &P, const std::vector<Z4cGeo> &regs, const Z4cK &K, int *flag,
                           hipStream_t st) {
   size_t r0 = 0;
@@ -1472,7 +1483,7 @@ static int launch_regions(const Z4cPtrs &P, const std::vector<Z4cGeo> &regs, con
     }
     J.blk0[J.n] = blk;
     if (blk == 0) continue;
-    z4c_rhs_tile<L, SH, NB, OCC><<<(unsigned)blk, dim3(L::TX, L::TY, L::TZ), 0, st>>>(P, J, K, flag);
+    z4c_rhs_tile<L, SH, NB, VEC, OCC><<<(unsigned)blk, dim3(L::TX, L::TY, L::TZ), 0, st>>>(P, J, K, flag);
     BH_HIP(hipGetLastError());
   }
   return BAMHIP_OK;
@@ -1553,42 +1564,45 @@ using RegionLists = std::vector<const std::vector<Z4cGeo> *>;
 using LaunchFn = int (*)(const Z4cPtrs &, const RegionLists &, const Z4cK &, int *, hipStream_t);
 struct Variant {
   const char *name;
-  bool dma;
+  bool dma;    // LDS-DMA staging: byte offsets within a field must fit 32 bits
+  bool pairs;  // 16-byte LDS-DMA units: needs 16-byte aligned element pairs
   LaunchFn fn;
 };
-template <class L, int NB, int OCC>
+template <class L, int NB, int VEC, int OCC>
 static int bulk_variant(const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, int *flag, hipStream_t st) {
-  return launch_regions<L, false, NB, OCC>(P, *R[0], K, flag, st);
+  return launch_regions<L, false, NB, VEC, OCC>(P, *R[0], K, flag, st);
 }
-template <int NB, int OCC>
+template <int NB, int VEC, int OCC>
 static int shell_variant(const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, int *flag, hipStream_t st) {
   int e;
-  if ((e = launch_regions<LSlabX, true, NB, OCC>(P, *R[0], K, flag, st))) return e;
-  if ((e = launch_regions<LSlabY, true, NB, OCC>(P, *R[1], K, flag, st))) return e;
-  return launch_regions<LSlabZ, true, NB, OCC>(P, *R[2], K, flag, st);
+  if ((e = launch_regions<LSlabX, true, NB, VEC, OCC>(P, *R[0], K, flag, st))) return e;
+  if ((e = launch_regions<LSlabY, true, NB, VEC, OCC>(P, *R[1], K, flag, st))) return e;
+  return launch_regions<LSlabZ, true, NB, VEC, OCC>(P, *R[2], K, flag, st);
 }
 template <int NB, int OCC>
 static int layer_variant(const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, int *flag, hipStream_t st) {
   int e;
-  if ((e = launch_regions<LLayX, true, NB, OCC>(P, *R[3], K, flag, st))) return e;
-  if ((e = launch_regions<LLayY, true, NB, OCC>(P, *R[4], K, flag, st))) return e;
-  return launch_regions<LLayZ, true, NB, OCC>(P, *R[5], K, flag, st);
+  if ((e = launch_regions<LLayX, true, NB, 1, OCC>(P, *R[3], K, flag, st))) return e;
+  if ((e = launch_regions<LLayY, true, NB, 1, OCC>(P, *R[4], K, flag, st))) return e;
+  return launch_regions<LLayZ, true, NB, 1, OCC>(P, *R[5], K, flag, st);
 }
-constexpr int kMaxVariants = 8;
+constexpr int kMaxVariants = 10;
 static const Variant kVariants[2][kMaxVariants] = {
-    {{"bulk 16x4x4 dma3", true, bulk_variant<LBulk, 3, 1>},
-     {"bulk 16x4x4 reg", false, bulk_variant<LBulk, 0, 1>},
-     {"bulk 16x4x4 dma4", true, bulk_variant<LBulk, 4, 1>},
-     {"bulk 32x4x2 dma3", true, bulk_variant<Lay<32, 4, 2>, 3, 1>},
-     {"bulk 16x8x2 dma3", true, bulk_variant<Lay<16, 8, 2>, 3, 1>},
-     {"bulk 16x4x4 dma3 occ2", true, bulk_variant<LBulk, 3, 2>},
-     {"bulk 16x4x4 dma4 occ2", true, bulk_variant<LBulk, 4, 2>},
-     {"bulk 32x4x2 dma3 occ2", true, bulk_variant<Lay<32, 4, 2>, 3, 2>}},
-    {{"shell layers dma3", true, layer_variant<3, 1>},
-     {"shell slabs reg", false, shell_variant<0, 1>},
-     {"shell slabs dma3", true, shell_variant<3, 1>},
-     {"shell layers dma4", true, layer_variant<4, 1>}}};
-static const int kNumVariants[2] = {8, 4};
+    {{"bulk 16x4x4 dma3", true, false, bulk_variant<LBulk, 3, 1, 1>},
+     {"bulk 16x4x4 reg", false, false, bulk_variant<LBulk, 0, 1, 1>},
+     {"bulk 16x4x4 dma4", true, false, bulk_variant<LBulk, 4, 1, 1>},
+     {"bulk 32x4x2 dma3", true, false, bulk_variant<Lay<32, 4, 2>, 3, 1, 1>},
+     {"bulk 16x8x2 dma3", true, false, bulk_variant<Lay<16, 8, 2>, 3, 1, 1>},
+     {"bulk 16x4x4 dma3 x4", true, true, bulk_variant<LBulk, 3, 4, 1>},
+     {"bulk 16x4x4 dma4 x4", true, true, bulk_variant<LBulk, 4, 4, 1>},
+     {"bulk 32x4x2 dma3 x4", true, true, bulk_variant<Lay<32, 4, 2>, 3, 4, 1>},
+     {"bulk 16x4x4 dma4 occ2", true, false, bulk_variant<LBulk, 4, 1, 2>}},
+    {{"shell layers dma3", true, false, layer_variant<3, 1>},
+     {"shell slabs reg", false, false, shell_variant<0, 1, 1>},
+     {"shell slabs dma3", true, false, shell_variant<3, 1, 1>},
+     {"shell layers dma4", true, false, layer_variant<4, 1>},
+     {"shell slabs dma3 x4", true, true, shell_variant<3, 4, 1>}}};
+static const int kNumVariants[2] = {9, 5};
 // variant 0 of each kind is the default; variant 1 needs no LDS-DMA
 
 struct TuneEntry {
@@ -1623,7 +1637,7 @@ static std::string region_signature(const RegionLists &R) {
 // the chosen variant has also been run once more, so the outputs are in place
 // and *done is set.
 static int choose_variant(int kind, const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, int *flag,
-                          bool repeatable, bool dma_ok, int *idx, bool *done) {
+                          bool repeatable, bool dma_ok, bool pairs_ok, int *idx, bool *done) {
   const Variant *V = kVariants[kind];
   const int nv = kNumVariants[kind];
   *done = false;
@@ -1631,6 +1645,8 @@ static int choose_variant(int kind, const Z4cPtrs &P, const RegionLists &R, cons
   if (*idx >= 0) {
     if (V[*idx].dma && !dma_ok)
       return set_error(BAMHIP_EINVAL, "z4c tile variant '%s' needs a level below 2^29 points", V[*idx].name);
+    if (V[*idx].pairs && !pairs_ok)
+      return set_error(BAMHIP_EINVAL, "z4c tile variant '%s' needs even box sizes and offsets", V[*idx].name);
     return BAMHIP_OK;
   }
   const std::string sig = region_signature(R);
@@ -1653,7 +1669,7 @@ static int choose_variant(int kind, const Z4cPtrs &P, const RegionLists &R, cons
   int e = BAMHIP_OK;
   for (int v = 0; v < nv && e == BAMHIP_OK; v++) {
     te.ms[v] = -1.0;
-    if (V[v].dma && !dma_ok) continue;
+    if ((V[v].dma && !dma_ok) || (V[v].pairs && !pairs_ok)) continue;
     if ((e = V[v].fn(P, R, K, flag, stream()))) break;
     BH_HIP(hipEventRecord(a, stream()));
     for (int rep = 0; rep < 2 && e == BAMHIP_OK; rep++) e = V[v].fn(P, R, K, flag, stream());
@@ -1726,8 +1742,16 @@ int z4c_rhs_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int *une
   const bool have_shell = !T.sx.empty() || !T.sy.empty() || !T.sz.empty();
   int ib = -1, is = -1;
   bool db = false, ds = false;
-  if (!T.bulk.empty() && (e = choose_variant(0, P, RB, K, lev->d_flag, repeatable, dma_ok, &ib, &db))) return e;
-  if (have_shell && (e = choose_variant(1, P, RS, K, lev->d_flag, repeatable, dma_ok, &is, &ds))) return e;
+  // 16-byte units hold element pairs (x, x + 1) with x even in every box:
+  // even box rows, box offsets and tile rows (first staged x = i0 - 3 even)
+  bool pairs_ok = true;
+  for (const auto *v : {&T.bulk, &T.sx, &T.sy, &T.sz})
+    for (const Z4cGeo &G : *v) pairs_ok = pairs_ok && G.m % 2 == 0 && G.noffset % 2 == 0 && (G.lo[0] - HALO) % 2 == 0;
+  if (!T.bulk.empty() &&
+      (e = choose_variant(0, P, RB, K, lev->d_flag, repeatable, dma_ok, pairs_ok, &ib, &db)))
+    return e;
+  if (have_shell && (e = choose_variant(1, P, RS, K, lev->d_flag, repeatable, dma_ok, pairs_ok, &is, &ds)))
+    return e;
   if (ib >= 0 && is >= 0 && !db && !ds && g_tune.overlap) {
     // the shell on the side stream beside the bulk: the two write disjoint
     // points, and the shell's few blocks fill the CUs the bulk's last wave of

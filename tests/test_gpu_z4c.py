@@ -154,6 +154,31 @@ def test_rhs_tile_variants_bitwise_equal_on_uneven_level():
     lev.free()
 
 
+def test_rhs_pair_variants_rejected_on_odd_rows():
+    """16-byte staging needs even box rows: on a 21-wide box a pinned pair
+    variant is refused, and the tuner skips it and still matches the oracle"""
+    h = 0.2
+    ob = O.make_box(21, 18, 16, -2.0, -1.7, -1.5, h, skip=(1,) * 6)
+    X = grid(ob)
+    u = puncture_z4c(X, masses=((1.0, (0.05, -0.02, 0.03)),))
+    cpu, _ = O.z4c_rhs(ob, O.z4c_params(), u)
+    lev = _level_for(ob, (B.PHYBOUND,) * 6, nvars=50)
+    lev.put_state(U, u.reshape(25, -1))
+    names = B.variants(B.BULK)
+    pair = [v for v, nm in enumerate(names) if nm.endswith("x4")][0]
+    try:
+        B.force_variant(B.BULK, pair)
+        with pytest.raises(B.BamhipError):
+            lev.z4c_rhs(B.z4c_params(), R, U)
+    finally:
+        B.force_variant(B.BULK, -1)
+    lev.z4c_rhs(B.z4c_params(), R, U)
+    err = _rel_err(lev.get_state(R).reshape(u.shape), cpu)
+    assert err.max() <= 1e-12, dict(zip(B.VAR_NAMES, err))
+    assert "x4=-1.0000ms" in B.tune_report()
+    lev.free()
+
+
 @pytest.mark.parametrize("params", [dict(advection_lopsided=0), dict(order_centered=2, order_advection=2),
                                     dict(advection_lopsided=2, order_advection=4)])
 def test_rhs_generic_orders(params):
