@@ -96,6 +96,8 @@ def lib():
     _sig(L, "bamhip_z4c_tune_mode", i, [i])
     _sig(L, "bamhip_z4c_tune_clear", i, [])
     _sig(L, "bamhip_z4c_set_overlap", i, [i])
+    _sig(L, "bamhip_z4c_set_rk_fusion", i, [i])
+    _sig(L, "bamhip_z4c_set_xcd_map", i, [i])
     _sig(L, "bamhip_z4c_variant_count", i, [i])
     _sig(L, "bamhip_z4c_variant_name", C.c_char_p, [i, i])
     _sig(L, "bamhip_z4c_force_variant", i, [i, i])
@@ -621,6 +623,17 @@ def tune_mode(mode):
 
 def tune_clear():
     check(lib().bamhip_z4c_tune_clear(), "bamhip_z4c_tune_clear")
+
+
+def set_rk_fusion(mode=2):
+    """RK4 stage update inside the RHS kernels: 0 never, 1 always, 2 on
+    levels of at least 2^22 points (default)"""
+    return lib().bamhip_z4c_set_rk_fusion(int(mode))
+
+
+def set_xcd_map(on=True):
+    """XCD-aware tile numbering in the RHS tile kernels"""
+    return lib().bamhip_z4c_set_xcd_map(1 if on else 0)
 
 
 def set_overlap(on=True):

@@ -65,7 +65,15 @@ struct RadInfo {
 };
 
 // one thread per surface point of one box, all 25 variables
-__global__ void radiative_kernel(Ptrs25 unew, Ptrs25 upre, Ptrs25 ucur, RadInfo ri, DevBox B, double c) {
+// rk = 0: unew = RHS (c == 0) or upre + c RHS at the boundary points; rk = 1
+// (an RK4 stage with the update fused in): uc += wc RHS and, when has_v,
+// unew = upre + wv RHS
+struct RkFuse {
+  Ptrs25 uc;
+  double wc, wv;
+  int rk, has_v;
+};
+__global__ void radiative_kernel(Ptrs25 unew, Ptrs25 upre, Ptrs25 ucur, RadInfo ri, DevBox B, double c, RkFuse F) {
   const int M = B.m, N = B.n, O = B.o;
   const long nface_x = (long)N * O, nface_y = (long)(M - 2) * O, nface_z = (long)(M - 2) * (N - 2);
   const long total = 2 * (nface_x + nface_y + nface_z);
@@ -132,8 +140,50 @@ __global__ void radiative_kernel(Ptrs25 unew, Ptrs25 upre, Ptrs25 ucur, RadInfo 
     double rhs = -sp * (u[q] - ri.var0[v]) / rr;
     rhs -= vx * del(u, vx, Mcc, mcc, pcc, Pcc) / (2 * B.dx) + vy * del(u, vy, cMc, cmc, cpc, cPc) / (2 * B.dy) +
            vz * del(u, vz, ccM, ccm, ccp, ccP) / (2 * B.dz);
-    unew.p[v][q] = (c != 0.0) ? upre.p[v][q] + c * rhs : rhs;
+    if (F.rk) {
+      F.uc.p[v][q] = F.uc.p[v][q] + F.wc * rhs;
+      if (F.has_v) unew.p[v][q] = upre.p[v][q] + F.wv * rhs;
+    } else {
+      unew.p[v][q] = (c != 0.0) ? upre.p[v][q] + c * rhs : rhs;
+    }
   }
+}
+
+// dst = src for all 25 variables at the points of one box that no RHS kernel
+// computes (the skip layers of width skip[f] at face f), enumerated as x slabs
+// (full yz planes), y slabs (interior x) and z slabs (interior x and y)
+__global__ void skip_copy_kernel(Ptrs25 dst, Ptrs25 src, DevBox B) {
+  const int M = B.m, N = B.n, O = B.o;
+  const int *s = B.skip;
+  const int ix = M - s[0] - s[1], iy = N - s[2] - s[3];
+  const long cx = (long)(s[0] + s[1]) * N * O, cy = (long)ix * (s[2] + s[3]) * O,
+             cz = (long)ix * iy * (s[4] + s[5]);
+  long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (r < cx) {
+    const int l = (int)(r % (s[0] + s[1]));
+    r /= (s[0] + s[1]);
+    i = l < s[0] ? l : M - s[1] + (l - s[0]);
+    j = (int)(r % N);
+    k = (int)(r / N);
+  } else if ((r -= cx) < cy) {
+    const int l = (int)(r % (s[2] + s[3]));
+    r /= (s[2] + s[3]);
+    j = l < s[2] ? l : N - s[3] + (l - s[2]);
+    i = s[0] + (int)(r % ix);
+    k = (int)(r / ix);
+  } else if ((r -= cy) < cz) {
+    const int l = (int)(r % (s[4] + s[5]));
+    r /= (s[4] + s[5]);
+    k = l < s[4] ? l : O - s[5] + (l - s[4]);
+    i = s[0] + (int)(r % ix);
+    j = s[2] + (int)(r / ix);
+  } else {
+    return;
+  }
+  const long q = B.noffset + i + (long)M * j + (long)M * N * k;
+#pragma unroll 5
+  for (int v = 0; v < 25; v++) dst.p[v][q] = src.p[v][q];
 }
 
 // periodic copy of g layers along direction d for nv variables
@@ -444,18 +494,13 @@ int bamhip_vladd(bamhip_level *lev, int n, const int *r, double ca, const int *a
   return BAMHIP_OK;
 }
 
-int bamhip_z4c_set_boundary(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, const int *upre,
-                            double c, const int *ucur, int pgh) {
-  if (!lev || !par) return set_error(BAMHIP_EINVAL, "bamhip_z4c_set_boundary: NULL argument");
-  Ptrs25 N, Pp, U;
-  int e;
-  if ((e = get25(lev, unew, 25, N, "bamhip_z4c_set_boundary(unew)"))) return e;
-  if ((e = get25(lev, ucur, 25, U, "bamhip_z4c_set_boundary(ucur)"))) return e;
-  if (c != 0.0) {
-    if ((e = get25(lev, upre, 25, Pp, "bamhip_z4c_set_boundary(upre)"))) return e;
-  } else {
-    Pp = U;
-  }
+}  // extern "C"
+
+// The boundary of one RHS evaluation or fused RK4 stage on every box:
+// radiative condition on PHYBOUND faces, periodic copy of pgh layers of the
+// lists in per_lists on PERBOUND face pairs.
+static int boundary_impl(bamhip_level *lev, const bamhip_z4c_params *par, Ptrs25 N, Ptrs25 Pp, Ptrs25 U, double c,
+                         const RkFuse &F, const Ptrs25 *per_lists, int nper, int pgh, const char *what) {
   RadInfo ri;
   double vg = 1.0;
   if (par->oploglapse) vg = sqrt(par->lapseharmonicf);
@@ -472,33 +517,84 @@ int bamhip_z4c_set_boundary(bamhip_level *lev, const bamhip_z4c_params *par, con
       anyper |= B.bflag[f] == BAMHIP_PERBOUND;
     }
     if (anyphy) {
-      if (B.m < 3 || B.n < 3 || B.o < 3)
-        return set_error(BAMHIP_EINVAL, "bamhip_z4c_set_boundary: box too small for the radiative condition");
+      if (B.m < 3 || B.n < 3 || B.o < 3) return set_error(BAMHIP_EINVAL, "%s: box too small for the radiative condition", what);
       const long total = 2L * ((long)B.n * B.o + (long)(B.m - 2) * B.o + (long)(B.m - 2) * (B.n - 2));
-      radiative_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream()>>>(N, Pp, U, ri, B, c);
+      radiative_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream()>>>(N, Pp, U, ri, B, c, F);
       BH_HIP(hipGetLastError());
     }
     if (anyper) {
-      if (pgh < 1) return set_error(BAMHIP_EINVAL, "bamhip_z4c_set_boundary: periodic faces need pgh >= 1");
+      if (pgh < 1) return set_error(BAMHIP_EINVAL, "%s: periodic faces need pgh >= 1", what);
       const int dims[3] = {B.m, B.n, B.o};
       int per[3];
       for (int d = 0; d < 3; d++) {
         per[d] = (B.bflag[2 * d] == BAMHIP_PERBOUND && B.bflag[2 * d + 1] == BAMHIP_PERBOUND);
         if (per[d] && dims[d] < 3 * pgh)
-          return set_error(BAMHIP_EINVAL, "bamhip_z4c_set_boundary: periodic direction %d has %d < 3*%d points", d,
-                           dims[d], pgh);
+          return set_error(BAMHIP_EINVAL, "%s: periodic direction %d has %d < 3*%d points", what, d, dims[d], pgh);
       }
       const long ix = per[0] ? B.m - 2 * pgh : B.m, iy = per[1] ? B.n - 2 * pgh : B.n;
       const long total = 2 * ((per[0] ? (long)pgh * B.n * B.o : 0) + (per[1] ? (long)pgh * ix * B.o : 0) +
                               (per[2] ? (long)pgh * ix * iy : 0));
       if (total > 0) {
         dim3 grid((unsigned)((total + 255) / 256), 25, 1);
-        periodic_all_kernel<<<grid, 256, 0, stream()>>>(N, B, pgh, per[0], per[1], per[2]);
-        BH_HIP(hipGetLastError());
+        for (int l = 0; l < nper; l++) {
+          periodic_all_kernel<<<grid, 256, 0, stream()>>>(per_lists[l], B, pgh, per[0], per[1], per[2]);
+          BH_HIP(hipGetLastError());
+        }
       }
     }
   }
   return BAMHIP_OK;
+}
+
+extern "C" {
+
+int bamhip_z4c_set_boundary(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, const int *upre,
+                            double c, const int *ucur, int pgh) {
+  if (!lev || !par) return set_error(BAMHIP_EINVAL, "bamhip_z4c_set_boundary: NULL argument");
+  Ptrs25 N, Pp, U;
+  int e;
+  if ((e = get25(lev, unew, 25, N, "bamhip_z4c_set_boundary(unew)"))) return e;
+  if ((e = get25(lev, ucur, 25, U, "bamhip_z4c_set_boundary(ucur)"))) return e;
+  if (c != 0.0) {
+    if ((e = get25(lev, upre, 25, Pp, "bamhip_z4c_set_boundary(upre)"))) return e;
+  } else {
+    Pp = U;
+  }
+  RkFuse F{};
+  return boundary_impl(lev, par, N, Pp, U, c, F, &N, 1, pgh, "bamhip_z4c_set_boundary");
+}
+
+int bamhip_z4c_bc_rk4_stage(bamhip_level *lev, const bamhip_z4c_params *par, const int *src, const int *uc,
+                            const int *up, const int *vout, double wc, double wv, int pgh) {
+  if (!lev || !par || !src || !uc || !up) return set_error(BAMHIP_EINVAL, "bamhip_z4c_bc_rk4_stage: NULL argument");
+  Ptrs25 U, Pp, V;
+  RkFuse F{};
+  int e;
+  if ((e = get25(lev, src, 25, U, "bamhip_z4c_bc_rk4_stage(src)"))) return e;
+  if ((e = get25(lev, uc, 25, F.uc, "bamhip_z4c_bc_rk4_stage(uc)"))) return e;
+  if ((e = get25(lev, up, 25, Pp, "bamhip_z4c_bc_rk4_stage(up)"))) return e;
+  F.rk = 1;
+  F.wc = wc;
+  F.wv = wv;
+  F.has_v = vout != nullptr;
+  if (vout) {
+    if ((e = get25(lev, vout, 25, V, "bamhip_z4c_bc_rk4_stage(vout)"))) return e;
+    // points no kernel computes: v = u_p + wv * 0
+    for (const DevBox &B : lev->boxes) {
+      const long ix = B.m - B.skip[0] - B.skip[1], iy = B.n - B.skip[2] - B.skip[3];
+      if (ix < 1 || iy < 1 || B.o - B.skip[4] - B.skip[5] < 1)
+        return set_error(BAMHIP_EINVAL, "bamhip_z4c_bc_rk4_stage: a box has no inner points");
+      const long total = (long)(B.skip[0] + B.skip[1]) * B.n * B.o + ix * (B.skip[2] + B.skip[3]) * B.o +
+                         ix * iy * (B.skip[4] + B.skip[5]);
+      if (total <= 0) continue;
+      skip_copy_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream()>>>(V, Pp, B);
+      BH_HIP(hipGetLastError());
+    }
+  } else {
+    V = Pp;
+  }
+  const Ptrs25 lists[2] = {F.uc, V};
+  return boundary_impl(lev, par, V, Pp, U, 0.0, F, lists, vout ? 2 : 1, pgh, "bamhip_z4c_bc_rk4_stage");
 }
 
 int bamhip_z4c_algcon(bamhip_level *lev, const int *g, const int *A, int subtractA, int normalizedetg) {

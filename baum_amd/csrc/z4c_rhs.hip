@@ -55,10 +55,14 @@ __device__ __forceinline__ void static_for(F &&f) {
 
 namespace bamhip {
 
+// u: the state the stencils read; r: output (the RHS, unew = upre + c RHS, or
+// in an RK4 stage the next stage's state); p: upre, or u_p in an RK4 stage;
+// c: the RK4 accumulator u_c (read and written at the point only)
 struct Z4cPtrs {
   const double *u[BAMHIP_Z4C_NVARS];
   double *r[BAMHIP_Z4C_NVARS];
   const double *p[BAMHIP_Z4C_NVARS];
+  double *c[BAMHIP_Z4C_NVARS];
 };
 
 struct Z4cGeo {
@@ -82,6 +86,10 @@ struct Z4cK {
   double chiDivFloor, chipsipower;
   double c;
   int addlinear;
+  // RK4 stage (rk.c:231-252) fused into the store: u_c += wc RHS and, when
+  // has_v, v_next = u_p + wv RHS
+  int rk, has_v;
+  double wc, wv;
 };
 
 // scheduling fence: keeps the machine scheduler from hoisting the loads of
@@ -200,6 +208,24 @@ struct Edge {
 // write one point: raw RHS, or upre + c*RHS with trace removal and det
 // normalisation (z4_rhs_movpunc_N.m:446-467)
 __device__ __forceinline__ void store_point(const Z4cPtrs &P, const Z4cK &K, long q, double *rhs) {
+  if (K.rk) {
+    // all loads before any store: the lists may alias as far as the compiler
+    // knows, so loads written after a store would each wait for it
+    double cv[25], pv[25];
+#pragma unroll
+    for (int v = 0; v < 25; v++) cv[v] = P.c[v][q];
+    if (K.has_v) {
+#pragma unroll
+      for (int v = 0; v < 25; v++) pv[v] = P.p[v][q];
+    }
+#pragma unroll
+    for (int v = 0; v < 25; v++) P.c[v][q] = cv[v] + K.wc * rhs[v];
+    if (K.has_v) {
+#pragma unroll
+      for (int v = 0; v < 25; v++) P.r[v][q] = pv[v] + K.wv * rhs[v];
+    }
+    return;
+  }
   if (K.addlinear) {
 #pragma unroll
     for (int v = 0; v < 25; v++) rhs[v] = P.p[v][q] + K.c * rhs[v];
@@ -871,7 +897,8 @@ struct Spacing {
 };
 struct TileJobs {
   int n;
-  int blk0[kMaxRegions + 1];
+  int blk0[kMaxRegions + 1];  // first tile of each region; blk0[n] = number of tiles
+  int xcd;                    // 1: XCD-aware numbering of the tiles (grid padded to a multiple of 8)
   RegionGeo reg[kMaxRegions];
   Spacing sp;
 };
@@ -897,9 +924,20 @@ struct StageBuf<L, 0, VEC> {
 template <class L, bool SH, int NB, int VEC, int OCC>
 __global__ void __launch_bounds__(NTHR, OCC) z4c_rhs_tile(Z4cPtrs P, TileJobs J, Z4cK K, int *flag) {
   __shared__ double tile[StageBuf<L, NB, VEC>::n];
+  // Tile of this block.  Workgroups are dispatched to the 8 XCDs round-robin
+  // (workgroup w to XCD w mod 8), each XCD with its own L2.  With xcd set,
+  // workgroup w takes tile (w mod 8) * per + w / 8, so every XCD works through
+  // one contiguous range of tiles and neighbouring tiles, which stage the same
+  // halo points, meet in the same L2.
+  int t = (int)blockIdx.x;
+  if (J.xcd) {
+    const int per = (int)gridDim.x / 8;
+    t = (t % 8) * per + t / 8;
+    if (t >= J.blk0[J.n]) return;  // padding workgroup (whole block, before any barrier)
+  }
   // region of this block (uniform): the regions' tiles are numbered in order
   int r = 0;
-  while (r + 1 < J.n && (int)blockIdx.x >= J.blk0[r + 1]) r++;
+  while (r + 1 < J.n && t >= J.blk0[r + 1]) r++;
   // copies: indexing the argument array by r through a reference would stage it in scratch
   const RegionGeo Rg = J.reg[r];
   Z4cGeo G;
@@ -923,7 +961,7 @@ __global__ void __launch_bounds__(NTHR, OCC) z4c_rhs_tile(Z4cPtrs P, TileJobs J,
     G.i4hh[a] = J.sp.i4hh[a];
   }
   G.edge_only = 0;
-  const int b = (int)blockIdx.x - J.blk0[r];
+  const int b = t - J.blk0[r];
   const int ntx = (G.hi[0] - G.lo[0] + L::TX) / L::TX, nty = (G.hi[1] - G.lo[1] + L::TY) / L::TY;
   const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
   const int tid = tx + L::TX * (ty + L::TY * tz);
@@ -1383,6 +1421,9 @@ static Z4cK make_k(const bamhip_z4c_params *p, double c) {
   K.chipsipower = p->chipsipower;
   K.c = c;
   K.addlinear = (c != 0.0);
+  K.rk = 0;
+  K.has_v = 0;
+  K.wc = K.wv = 0.0;
   return K;
 }
 
@@ -1441,6 +1482,8 @@ static int ntiles(const Z4cGeo &G) {
 }
 
 // one launch per kMaxRegions regions of a layout
+static int g_xcd_map = 1;  // XCD-aware tile numbering in the tile kernels
+
 static bool same_spacing(const Z4cGeo &a, const Z4cGeo &b) {
   return a.h[0] == b.h[0] && a.h[1] == b.h[1] && a.h[2] == b.h[2];
 }
@@ -1483,7 +1526,9 @@ static int launch_regions(const Z4cPtrs &P, const std::vector<Z4cGeo> &regs, con
     }
     J.blk0[J.n] = blk;
     if (blk == 0) continue;
-    z4c_rhs_tile<L, SH, NB, VEC, OCC><<<(unsigned)blk, dim3(L::TX, L::TY, L::TZ), 0, st>>>(P, J, K, flag);
+    J.xcd = g_xcd_map;
+    const unsigned grid = g_xcd_map ? (unsigned)(8 * ((blk + 7) / 8)) : (unsigned)blk;
+    z4c_rhs_tile<L, SH, NB, VEC, OCC><<<grid, dim3(L::TX, L::TY, L::TZ), 0, st>>>(P, J, K, flag);
     BH_HIP(hipGetLastError());
   }
   return BAMHIP_OK;
@@ -1693,50 +1738,37 @@ static int choose_variant(int kind, const Z4cPtrs &P, const RegionLists &R, cons
   return V[te.choice].fn(P, R, K, flag, stream());
 }
 
-int z4c_rhs_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, const int *upre, double c,
-                 const int *ucur, bool check_finite) {
-  if (!lev || !par) return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: NULL argument");
-  if (par->order_centered != 2 && par->order_centered != 4)
-    return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: order_centered=%d not implemented (2, 4)", par->order_centered);
-  if (par->order_advection != 2 && par->order_advection != 4)
-    return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: order_advection=%d not implemented (2, 4)",
-                     par->order_advection);
-  if (par->advection_lopsided < 0 || par->advection_lopsided > 2)
-    return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: advection_lopsided=%d", par->advection_lopsided);
-  if (par->order_dissipation != 0 && par->order_dissipation != 4)
-    return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: order_dissipation=%d not implemented (0, 4)",
-                     par->order_dissipation);
-  Z4cPtrs P;
+// The tile and edge kernels of one RHS evaluation (plain, addlinear or fused
+// RK4 stage, as K says) over all boxes of a level.  A fused stage rewrites u_c,
+// so its kernels cannot be timed on the real lists: the tuner then times the
+// plain RHS into scratch (when given) and the stage runs the chosen variant.
+static int run_rhs(bamhip_level *lev, const Z4cPtrs &P, const Z4cK &K, bool check_finite, double *const *scratch) {
   int e;
-  if ((e = gather_ptrs(lev, ucur, 25, const_cast<double **>(P.u), "bamhip_z4c_rhs(ucur)"))) return e;
-  if ((e = gather_ptrs(lev, unew, 25, P.r, "bamhip_z4c_rhs(unew)"))) return e;
-  if (c != 0.0) {
-    if ((e = gather_ptrs(lev, upre, 25, const_cast<double **>(P.p), "bamhip_z4c_rhs(upre)"))) return e;
-  } else {
-    for (int v = 0; v < 25; v++) P.p[v] = P.u[v];
-  }
-  for (int v = 0; v < 25; v++)
-    for (int w = 0; w < 25; w++)
-      if (P.r[v] == P.u[w])
-        return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: unew[%d] aliases ucur[%d] (stencils need the old data)", v, w);
-  // the stencil reach (3 with lop-sided advection, 4 with one-sided) must stay
-  // inside every box
-  for (const DevBox &B : lev->boxes)
-    for (int f = 0; f < 6; f++)
-      if (B.skip[f] < 1) return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: box face %d has no flagged layer", f);
-  const Z4cK K = make_k(par, c);
   if (check_finite) set_int<<<1, 1, 0, stream()>>>(lev->d_flag, 0);
   const bool fast = (K.order_centered == 4 && K.order_advection == 4 && K.lopsided == 1 &&
                      (K.order_dissipation == 4 || K.order_dissipation == 0));
   LevelTiles T;
   for (const DevBox &B : lev->boxes)
     if ((e = plan_box(B, P, K, lev->d_flag, fast, T))) return e;
-  // repeated launches (tuning) give the same result unless unew overwrites
-  // what the kernel reads
-  bool repeatable = true;
+  // repeated launches (tuning) give the same result unless the kernels write
+  // what they read
+  bool repeatable = !K.rk;
   for (int v = 0; v < 25 && repeatable; v++)
     for (int w = 0; w < 25; w++)
       if (P.r[v] == P.p[w]) repeatable = false;
+  Z4cPtrs PT = P;  // lists the tuner runs on
+  Z4cK KT = K;
+  if (K.rk && scratch) {
+    for (int v = 0; v < 25; v++) {
+      PT.r[v] = scratch[v];
+      PT.p[v] = P.u[v];
+    }
+    KT.rk = 0;
+    KT.has_v = 0;
+    KT.addlinear = 0;
+    KT.c = 0.0;
+    repeatable = true;
+  }
   const bool dma_ok = (long)lev->npoints * 8 < (1L << 32);
   const RegionLists RB = {&T.bulk}, RS = {&T.sx, &T.sy, &T.sz, &T.lx, &T.ly, &T.lz};
   const bool have_shell = !T.sx.empty() || !T.sy.empty() || !T.sz.empty();
@@ -1748,10 +1780,11 @@ int z4c_rhs_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int *une
   for (const auto *v : {&T.bulk, &T.sx, &T.sy, &T.sz})
     for (const Z4cGeo &G : *v) pairs_ok = pairs_ok && G.m % 2 == 0 && G.noffset % 2 == 0 && (G.lo[0] - HALO) % 2 == 0;
   if (!T.bulk.empty() &&
-      (e = choose_variant(0, P, RB, K, lev->d_flag, repeatable, dma_ok, pairs_ok, &ib, &db)))
+      (e = choose_variant(0, PT, RB, KT, lev->d_flag, repeatable, dma_ok, pairs_ok, &ib, &db)))
     return e;
-  if (have_shell && (e = choose_variant(1, P, RS, K, lev->d_flag, repeatable, dma_ok, pairs_ok, &is, &ds)))
+  if (have_shell && (e = choose_variant(1, PT, RS, KT, lev->d_flag, repeatable, dma_ok, pairs_ok, &is, &ds)))
     return e;
+  if (K.rk) db = ds = false;  // the tuner ran the plain RHS; the stage itself is still to run
   if (ib >= 0 && is >= 0 && !db && !ds && g_tune.overlap) {
     // the shell on the side stream beside the bulk: the two write disjoint
     // points, and the shell's few blocks fill the CUs the bulk's last wave of
@@ -1780,7 +1813,105 @@ int z4c_rhs_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int *une
   return BAMHIP_OK;
 }
 
+int z4c_rhs_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, const int *upre, double c,
+                 const int *ucur, bool check_finite) {
+  if (!lev || !par) return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: NULL argument");
+  if (par->order_centered != 2 && par->order_centered != 4)
+    return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: order_centered=%d not implemented (2, 4)", par->order_centered);
+  if (par->order_advection != 2 && par->order_advection != 4)
+    return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: order_advection=%d not implemented (2, 4)",
+                     par->order_advection);
+  if (par->advection_lopsided < 0 || par->advection_lopsided > 2)
+    return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: advection_lopsided=%d", par->advection_lopsided);
+  if (par->order_dissipation != 0 && par->order_dissipation != 4)
+    return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: order_dissipation=%d not implemented (0, 4)",
+                     par->order_dissipation);
+  Z4cPtrs P;
+  int e;
+  if ((e = gather_ptrs(lev, ucur, 25, const_cast<double **>(P.u), "bamhip_z4c_rhs(ucur)"))) return e;
+  if ((e = gather_ptrs(lev, unew, 25, P.r, "bamhip_z4c_rhs(unew)"))) return e;
+  if (c != 0.0) {
+    if ((e = gather_ptrs(lev, upre, 25, const_cast<double **>(P.p), "bamhip_z4c_rhs(upre)"))) return e;
+  } else {
+    for (int v = 0; v < 25; v++) P.p[v] = P.u[v];
+  }
+  for (int v = 0; v < 25; v++) P.c[v] = nullptr;
+  for (int v = 0; v < 25; v++)
+    for (int w = 0; w < 25; w++)
+      if (P.r[v] == P.u[w])
+        return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: unew[%d] aliases ucur[%d] (stencils need the old data)", v, w);
+  // the stencil reach (3 with lop-sided advection, 4 with one-sided) must stay
+  // inside every box
+  for (const DevBox &B : lev->boxes)
+    for (int f = 0; f < 6; f++)
+      if (B.skip[f] < 1) return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs: box face %d has no flagged layer", f);
+  const Z4cK K = make_k(par, c);
+  return run_rhs(lev, P, K, check_finite, nullptr);
+}
+
+static int check_params(const bamhip_z4c_params *par, const char *what) {
+  if (par->order_centered != 2 && par->order_centered != 4)
+    return set_error(BAMHIP_EINVAL, "%s: order_centered=%d not implemented (2, 4)", what, par->order_centered);
+  if (par->order_advection != 2 && par->order_advection != 4)
+    return set_error(BAMHIP_EINVAL, "%s: order_advection=%d not implemented (2, 4)", what, par->order_advection);
+  if (par->advection_lopsided < 0 || par->advection_lopsided > 2)
+    return set_error(BAMHIP_EINVAL, "%s: advection_lopsided=%d", what, par->advection_lopsided);
+  if (par->order_dissipation != 0 && par->order_dissipation != 4)
+    return set_error(BAMHIP_EINVAL, "%s: order_dissipation=%d not implemented (0, 4)", what, par->order_dissipation);
+  return BAMHIP_OK;
+}
+
+// One RK4 stage with the update fused into the RHS kernels' store: at every
+// computed point u_c += wc RHS(src) and (vout != NULL) vout = u_p + wv RHS(src).
+// src is read by the stencils, so neither u_c nor vout may alias it; u_c is
+// read and written at the point only.  scratch (optional): 25 variables the
+// tuner may use as a plain RHS output the first time a box set is seen.
+int z4c_rhs_rk4_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int *src, const int *uc, const int *up,
+                     const int *vout, double wc, double wv, const int *scratch) {
+  if (!lev || !par || !src || !uc || !up) return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs_rk4_stage: NULL argument");
+  int e;
+  if ((e = check_params(par, "bamhip_z4c_rhs_rk4_stage"))) return e;
+  Z4cPtrs P;
+  if ((e = gather_ptrs(lev, src, 25, const_cast<double **>(P.u), "bamhip_z4c_rhs_rk4_stage(src)"))) return e;
+  if ((e = gather_ptrs(lev, uc, 25, P.c, "bamhip_z4c_rhs_rk4_stage(uc)"))) return e;
+  if ((e = gather_ptrs(lev, up, 25, const_cast<double **>(P.p), "bamhip_z4c_rhs_rk4_stage(up)"))) return e;
+  if (vout) {
+    if ((e = gather_ptrs(lev, vout, 25, P.r, "bamhip_z4c_rhs_rk4_stage(vout)"))) return e;
+  } else {
+    for (int v = 0; v < 25; v++) P.r[v] = nullptr;
+  }
+  for (int v = 0; v < 25; v++)
+    for (int w = 0; w < 25; w++) {
+      if (P.c[v] == P.u[w] || (vout && (P.r[v] == P.u[w] || P.r[v] == P.c[w] || P.r[v] == P.p[w])))
+        return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs_rk4_stage: uc / vout alias src, up or each other");
+    }
+  for (const DevBox &B : lev->boxes)
+    for (int f = 0; f < 6; f++)
+      if (B.skip[f] < 1)
+        return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs_rk4_stage: box face %d has no flagged layer", f);
+  double *scr[BAMHIP_Z4C_NVARS];
+  if (scratch) {
+    if ((e = gather_ptrs(lev, scratch, 25, scr, "bamhip_z4c_rhs_rk4_stage(scratch)"))) return e;
+    for (int v = 0; v < 25; v++)
+      for (int w = 0; w < 25; w++)
+        if (scr[v] == P.u[w] || scr[v] == P.c[w] || scr[v] == P.p[w] || (vout && scr[v] == P.r[w]))
+          return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs_rk4_stage: scratch aliases another list");
+  }
+  Z4cK K = make_k(par, 0.0);
+  K.rk = 1;
+  K.has_v = vout ? 1 : 0;
+  K.wc = wc;
+  K.wv = wv;
+  return run_rhs(lev, P, K, lev->check_mode != 0, scratch ? scr : nullptr);
+}
+
 }  // namespace bamhip
+
+extern "C" int bamhip_z4c_rhs_rk4_stage(bamhip_level *lev, const bamhip_z4c_params *par, const int *src,
+                                        const int *uc, const int *up, const int *vout, double wc, double wv,
+                                        const int *scratch) {
+  return bamhip::z4c_rhs_rk4_impl(lev, par, src, uc, up, vout, wc, wv, scratch);
+}
 
 extern "C" int bamhip_z4c_rhs(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, const int *upre,
                               double c, const int *ucur) {
@@ -1797,6 +1928,12 @@ extern "C" int bamhip_z4c_tune_mode(int mode) {
 extern "C" int bamhip_z4c_set_overlap(int on) {
   const int old = bamhip::g_tune.overlap;
   bamhip::g_tune.overlap = on != 0;
+  return old;
+}
+
+extern "C" int bamhip_z4c_set_xcd_map(int on) {
+  const int old = bamhip::g_xcd_map;
+  bamhip::g_xcd_map = on != 0;
   return old;
 }
 

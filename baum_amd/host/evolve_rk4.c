@@ -10,6 +10,22 @@
  */
 #include "../../include/bamhip.h"
 
+/* 0: separate update pass; 1: update fused into the RHS kernels; 2 (default):
+ * fused on levels of at least 2^22 points.  The fused store adds the u_c and
+ * u_p reads and the u_c / v writes to every RHS block's end: on large levels
+ * that costs less than the separate pass, on small levels (a few thousand
+ * blocks, where the time goes in rounds of blocks) it costs more. */
+static int g_fused = 2;
+#define BAMHIP_FUSE_MIN_POINTS (1L << 22)
+
+int bamhip_z4c_set_rk_fusion(int mode)
+{
+  const int old = g_fused;
+  if (mode < 0 || mode > 2) return BAMHIP_EINVAL;
+  g_fused = mode;
+  return old;
+}
+
 int bamhip_z4c_rk4_stages(bamhip_level *lev, const bamhip_z4c_params *par, double dt, const int *uc, const int *up,
                           const int *r, const int *v, int pgh)
 {
@@ -19,6 +35,22 @@ int bamhip_z4c_rk4_stages(bamhip_level *lev, const bamhip_z4c_params *par, doubl
   int e, s;
   /* u_p = u_c */
   if ((e = bamhip_vladd(lev, BAMHIP_Z4C_NVARS, up, 1.0, uc, 0.0, 0))) return e;
+  long npoints = 0;
+  if ((e = bamhip_level_npoints(lev, &npoints))) return e;
+  if (g_fused == 1 || (g_fused == 2 && npoints >= BAMHIP_FUSE_MIN_POINTS)) {
+    /* The update inside the RHS kernels' store.  u_c is written at each
+     * point while the stencils read the stage state, so stage 1 reads u_p
+     * (equal to u_c at this point) and the stage states alternate between v
+     * and r: stage s reads src[s] and writes dst[s]. */
+    const int *src[4] = {up, v, r, v};
+    const int *dst[4] = {v, r, v, 0};
+    for (s = 0; s < 4; s++) {
+      if ((e = bamhip_z4c_rhs_rk4_stage(lev, par, src[s], uc, up, dst[s], wc[s], wv[s], s == 0 ? r : 0)))
+        return e;
+      if ((e = bamhip_z4c_bc_rk4_stage(lev, par, src[s], uc, up, dst[s], wc[s], wv[s], pgh))) return e;
+    }
+    return BAMHIP_OK;
+  }
   for (s = 0; s < 4; s++) {
     const int *src = (s == 0) ? uc : v;
     if ((e = bamhip_z4c_rhs(lev, par, r, 0, 0.0, src))) return e;

@@ -133,6 +133,11 @@ int bamhip_z4c_tune_mode(int mode);
 /* on != 0 (default): the shell kernels run on a second stream concurrently
  * with the bulk kernel (both in a captured graph too); returns the old value */
 int bamhip_z4c_set_overlap(int on);
+/* on != 0 (default): the tile kernels number their tiles so that each of the
+ * 8 XCDs (workgroup w runs on XCD w mod 8) takes one contiguous range of
+ * tiles, whose shared halo points then hit in that XCD's L2; returns the old
+ * value */
+int bamhip_z4c_set_xcd_map(int on);
 int bamhip_z4c_tune_clear(void);
 int bamhip_z4c_variant_count(int kind);
 const char *bamhip_z4c_variant_name(int kind, int idx);
@@ -365,6 +370,26 @@ int bamhip_fmr_nonfinite(const bamhip_fmr_hierarchy *H, int reset, int *flag);
 int bamhip_z4c_rk4_stages(bamhip_level *lev, const bamhip_z4c_params *par, double dt, const int *uc, const int *up,
                           const int *r, const int *v, int pgh);
 int bamhip_z4c_algcon_list(bamhip_level *lev, const bamhip_z4c_params *par, const int *uc);
+
+/* One RK4 stage of evolve_rk_rk4 (rk.c:231-252: evolve_rhs, set_boundary,
+ * vladdto, vladd) with the update fused into the RHS kernels: at every point
+ * uc += wc * RHS(src) and, when vout != NULL, vout = up + wv * RHS(src); the
+ * boundary call does the same on PHYBOUND faces (radiative RHS), sets
+ * vout = up at the other points no RHS kernel computes, and copies the
+ * periodic layers of uc and vout.  src must alias neither uc nor vout.
+ * scratch (optional, 25 indices): a list the tile-kernel tuner may overwrite
+ * with a plain RHS the first time a set of boxes is seen.  With fusion
+ * selected, bamhip_z4c_rk4_stages runs stage 1 from up (= uc) and alternates
+ * the stage states between v and r; results are bitwise those of the unfused
+ * stages when the periodic layers of uc start consistent.
+ * bamhip_z4c_set_rk_fusion(mode): 0 never, 1 always, 2 (default) on levels of
+ * at least 2^22 points; returns the previous mode. */
+int bamhip_z4c_rhs_rk4_stage(bamhip_level *lev, const bamhip_z4c_params *par, const int *src,
+                             const int *uc, const int *up, const int *vout, double wc, double wv,
+                             const int *scratch);
+int bamhip_z4c_bc_rk4_stage(bamhip_level *lev, const bamhip_z4c_params *par, const int *src,
+                            const int *uc, const int *up, const int *vout, double wc, double wv, int pgh);
+int bamhip_z4c_set_rk_fusion(int mode);
 
 /* --------------------------------------------------------------- graphs */
 /* Capture the device work issued on the library stream between begin and

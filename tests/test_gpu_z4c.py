@@ -137,6 +137,10 @@ def test_rhs_tile_variants_bitwise_equal_on_uneven_level():
                 results[name] = lev.get_state(R)
             B.force_variant(kind, -1)
             B.force_variant(other, -1)
+        B.set_xcd_map(False)
+        lev.z4c_rhs(par, R, U)
+        results["plain tile order"] = lev.get_state(R)
+        B.set_xcd_map(True)
         B.tune_clear()
         B.tune_mode(1)
         lev.z4c_rhs(par, R, U)
@@ -145,6 +149,7 @@ def test_rhs_tile_variants_bitwise_equal_on_uneven_level():
     finally:
         B.force_variant(B.BULK, -1)
         B.force_variant(B.SHELL, -1)
+        B.set_xcd_map(True)
     assert report.count("\n") == 2, report
     first = next(iter(results.values()))
     for name, got in results.items():
@@ -262,6 +267,51 @@ def test_rk4_puncture_box_radiative():
     err = _rel_err(gpu - u0, cpu - u0)
     assert np.all(np.isfinite(gpu))
     assert err.max() <= 1e-11, dict(zip(B.VAR_NAMES, err))
+
+
+@pytest.mark.parametrize("case", ["radiative", "periodic", "two_boxes"])
+def test_fused_rk4_stages_bitwise_equal_unfused(case):
+    """RK4 steps with the stage update fused into the RHS and boundary kernels
+    give bitwise the state of the separate RHS / boundary / update passes"""
+    if case == "periodic":
+        g, nin = 3, 20
+        n = nin + 2 * g
+        h = 1.0 / 12
+        ob = O.make_box(n, n, n, -(n - 1) * h / 2, -(n - 1) * h / 2, -(n - 1) * h / 2, h, skip=(g,) * 6)
+        rng = np.random.default_rng(77)
+        u0 = flat_z4c((n, n, n)) + rng.uniform(-1e-6, 1e-6, (25, n, n, n))
+        O.periodic_fill(ob, u0, g)
+        boxes = [B.make_box(n, n, n, ob.x0, ob.y0, ob.z0, h, skip=(g,) * 6, bflag=(B.PERBOUND,) * 6)]
+        state, pgh, dt = u0.reshape(25, -1), g, 0.25 * h
+    else:
+        h = 0.5
+        specs = [(22, 20, 18, (-5.25, -4.7, -4.3))]
+        if case == "two_boxes":
+            specs.append((16, 18, 14, (6.0, -4.0, -3.0)))
+        boxes, us = [], []
+        for m, nn, o, x0 in specs:
+            ob = O.make_box(m, nn, o, *x0, h, skip=(1,) * 6)
+            us.append(puncture_z4c(grid(ob), masses=((1.0, (0.05, -0.02, 0.03)),)).reshape(25, -1))
+            flag = B.PHYBOUND if case == "radiative" else B.REFBOUND
+            boxes.append(B.make_box(m, nn, o, *x0, h, skip=(1,) * 6, bflag=(flag,) * 6))
+        state, pgh, dt = np.concatenate(us, axis=1), 0, 0.25 * h
+    out = []
+    try:
+        for fused in (0, 1):
+            B.set_rk_fusion(fused)
+            lev = B.Level(boxes, 100)
+            lev.enable(range(100))
+            for v in range(100):
+                lev.fill(v, 0.0)
+            lev.put_state(U, state)
+            for _ in range(3):
+                lev.z4c_rk4_step(B.z4c_params(), dt, U, P, R, V, pgh=pgh)
+            out.append(lev.get_state(U))
+            lev.free()
+    finally:
+        B.set_rk_fusion(2)
+    assert np.abs(out[0] - state).max() > 1e-9
+    np.testing.assert_array_equal(out[1], out[0])
 
 
 def test_constraint_norms_tracked_over_time():
