@@ -446,7 +446,13 @@ __global__ void __launch_bounds__(64) c2p_learned_kernel(C2PArrays A, C2PK P, Ml
                                                          int *fallbacks) {
   // one wavefront per block: the barriers only order this wave's LDS traffic
   __shared__ __attribute__((aligned(16))) __bf16 h1[64 * kHStr];
+  __shared__ __attribute__((aligned(16))) float b2[kMlpH], w3[kMlpH], w1b1[4 * kMlpH];
   const int l = threadIdx.x;
+  b2[l] = M.p[kOffB2 + l];
+  w3[l] = M.wq[192 + l];
+  for (int j = 0; j < 3; j++) w1b1[3 * l + j] = M.wq[3 * l + j];
+  w1b1[192 + l] = M.p[kOffB1 + l];
+  __syncthreads();
   int nfb = 0;
   for (long base = (long)blockIdx.x * 64; base < n; base += (long)gridDim.x * 64) {
     const long q = base + l;
@@ -461,10 +467,9 @@ __global__ void __launch_bounds__(64) c2p_learned_kernel(C2PArrays A, C2PK P, Ml
       valid = mlp_features(M, u.D, u.S2, u.T, th, x);
       if (!valid) x[0] = x[1] = x[2] = 0.f;
     }
-    mlp_layer1(M, x, h1 + l * kHStr);
+    mlp_layer1_from(w1b1, x, h1 + l * kHStr);
     __syncthreads();
-    f32x16 c[2][2];
-    const float yn = mlp_layer23(M, h1, c);
+    const float yn = mlp_infer23(M, h1, b2, w3);
     __syncthreads();
     if (act) {
       double rho = 0., eps = 0., p = 0.;

@@ -82,7 +82,7 @@ __global__ void mlp_train_kernel(MlpView M, const float *__restrict__ feat, cons
   for (int k = 0; k < kMlpH; k++) h1t[k * kHStr + l] = h1[l * kHStr + k];
   __syncthreads();
   f32x16 c[2][2];
-  const float y = mlp_layer23(M, h1, c);
+  const float y = mlp_layer23(M, h1, c, M.p + kOffB2, M.wq + 192);
   const float r = valid ? y - t : 0.f;
   const float dy = 2.f * r * inv_batch;
   float lsum = r * r;
@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(256) mlp_predict_kernel(MlpView M, const float
     mlp_layer1(M, x, h1[w] + l * kHStr);
     __syncthreads();
     f32x16 c[2][2];
-    const float yn = mlp_layer23(M, h1[w], c);
+    const float yn = mlp_layer23(M, h1[w], c, M.p + kOffB2, M.wq + 192);
     if (q < n) y[q] = M.ys + M.yk * yn;
     __syncthreads();
   }

@@ -89,10 +89,31 @@ __device__ __forceinline__ void mlp_layer1(const MlpView &M, const float x[3], _
   }
 }
 
+// the same with W1 (bf16-rounded, [k][3]) and b1 read from w1b1 = {W1, b1}
+// (256 floats, an LDS copy in the inference kernel): the pre-activation of
+// unit k is formed in mlp_z1's order, so the values are those of mlp_layer1
+__device__ __forceinline__ void mlp_layer1_from(const float *w1b1, const float x[3], __bf16 *row) {
+#pragma unroll
+  for (int k0 = 0; k0 < kMlpH; k0 += 8) {
+    bf16x8 h;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int k = k0 + e;
+      const float z = fmaf(w1b1[3 * k + 2], x[2], fmaf(w1b1[3 * k + 1], x[1], fmaf(w1b1[3 * k], x[0], w1b1[192 + k])));
+      h[e] = (__bf16)fmaxf(z, 0.f);
+    }
+    *(bf16x8 *)(row + k0) = h;
+  }
+}
+
 // layers 2 and 3 for the 64 points whose bf16 rows are in h1s (row stride
 // kHStr).  c receives relu(Z2^T) tiles: c[m][n] holds features 32m.. and
 // points 32n..; the return value is the network output for point lane.
-__device__ __forceinline__ float mlp_layer23(const MlpView &M, const __bf16 *h1s, f32x16 c[2][2]) {
+// b2, w3: the layer-2 bias and the bf16-rounded W3 (64 floats each, 16-byte
+// aligned; LDS copies in the inference kernel).  Each lane reads them in
+// groups of four consecutive features (C rows 8q + 4 (lane/32) + 0..3).
+__device__ __forceinline__ float mlp_layer23(const MlpView &M, const __bf16 *h1s, f32x16 c[2][2], const float *b2,
+                                             const float *w3) {
   const int l = threadIdx.x & 63, lr = l % 32, lh = l / 32;
 #pragma unroll
   for (int m = 0; m < 2; m++)
@@ -116,18 +137,74 @@ __device__ __forceinline__ float mlp_layer23(const MlpView &M, const __bf16 *h1s
 #pragma unroll
   for (int m = 0; m < 2; m++)
 #pragma unroll
-    for (int n = 0; n < 2; n++)
+    for (int g = 0; g < 4; g++) {
+      const int f0 = 32 * m + mfma_row(4 * g, l);  // rows of v = 4g .. 4g + 3 are f0 .. f0 + 3
+      const float4 bb = *(const float4 *)(b2 + f0), ww = *(const float4 *)(w3 + f0);
+      const float bv[4] = {bb.x, bb.y, bb.z, bb.w}, wv[4] = {ww.x, ww.y, ww.z, ww.w};
 #pragma unroll
-      for (int v = 0; v < 16; v++) {
-        const int f = 32 * m + mfma_row(v, l);
-        const float h = fmaxf(c[m][n][v] + M.p[kOffB2 + f], 0.f);
-        c[m][n][v] = h;
-        part[n] = fmaf(h, M.wq[192 + f], part[n]);
-      }
+      for (int n = 0; n < 2; n++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const int v = 4 * g + e;
+          const float h = fmaxf(c[m][n][v] + bv[e], 0.f);
+          c[m][n][v] = h;
+          part[n] = fmaf(h, wv[e], part[n]);
+        }
+    }
 #pragma unroll
   for (int n = 0; n < 2; n++) part[n] += __shfl_xor(part[n], 32);
   // lanes 0..31 own points 0..31 (tile n = 0), lanes 32..63 points 32..63
   return (lh == 0 ? part[0] : part[1]) + M.p[kOffB3];
+}
+
+// Inference-only layers 2 and 3, one half of the points (n) at a time, so
+// that only two accumulator tiles are live (fewer registers, more waves per
+// SIMD).  The accumulators start at the layer-2 bias instead of adding it
+// after the contraction (the FP32 sums round differently from mlp_layer23 in
+// the last bits).
+__device__ __forceinline__ float mlp_infer23(const MlpView &M, const __bf16 *h1s, const float *b2, const float *w3) {
+  const int l = threadIdx.x & 63, lr = l % 32, lh = l / 32;
+  float part0 = 0.f, part1 = 0.f;
+#pragma unroll 1
+  for (int n = 0; n < 2; n++) {
+    float acc = 0.f;
+    f32x16 c[2];
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        const float4 bb = *(const float4 *)(b2 + 32 * m + mfma_row(4 * g, l));
+        c[m][4 * g] = bb.x;
+        c[m][4 * g + 1] = bb.y;
+        c[m][4 * g + 2] = bb.z;
+        c[m][4 * g + 3] = bb.w;
+      }
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const bf16x8 b = *(const bf16x8 *)(h1s + (32 * n + lr) * kHStr + 16 * s + 8 * lh);
+#pragma unroll
+      for (int m = 0; m < 2; m++) {
+        const bf16x8 a = *(const bf16x8 *)(M.w2 + (32 * m + lr) * kMlpH + 16 * s + 8 * lh);
+        c[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c[m], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        const float4 ww = *(const float4 *)(w3 + 32 * m + mfma_row(4 * g, l));
+        const float wv[4] = {ww.x, ww.y, ww.z, ww.w};
+#pragma unroll
+        for (int e = 0; e < 4; e++) acc = fmaf(fmaxf(c[m][4 * g + e], 0.f), wv[e], acc);
+      }
+    if (n == 0)
+      part0 = acc;
+    else
+      part1 = acc;
+  }
+  part0 += __shfl_xor(part0, 32);
+  part1 += __shfl_xor(part1, 32);
+  return (lh == 0 ? part0 : part1) + M.p[kOffB3];
 }
 
 }  // namespace bamhip
