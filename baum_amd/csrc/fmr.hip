@@ -56,7 +56,8 @@ struct RJob {
   long foff, coff;
   int fm, fmn, cm, cmn;
   int icb[6], ifb[6];
-  int ncx, ncy;
+  int ncx, ncy, ncz;
+  int ntx, nty, ntz;        // tiles of the LDS kernel per direction
   long npts, blk0;
 };
 struct RArgs {
@@ -177,6 +178,65 @@ __global__ void __launch_bounds__(kBlock) restrict_kernel(RArgs a, VarPtrs vp) {
   }
   vp.dst[v][J.coff + (J.icb[0] + cx) + (long)(J.icb[2] + cy) * J.cm + (long)(J.icb[4] + cz) * J.cmn] = sum;
 }
+
+// Restriction of a tile of RTX x RTY x RTZ coarse points through LDS: the
+// fine footprint is staged once, then the z sums of every fine (x, y) column,
+// the y sums and the x sums are formed as three sweeps, as the reference
+// does (:1108-1196).  Each coarse value is the same sequence of products and
+// sums as in restrict_kernel (a z sum e(ax, by) per fine column, then
+// dv(ax) = 0 + c0 e(ax, 0) + ..., then 0 + c0 dv(0) + ...), so the results
+// are bitwise equal; the fine data is read from HBM once instead of N^3 / 8
+// times.
+constexpr int RTX = 8, RTY = 8, RTZ = 4;
+template <int N>
+__global__ void __launch_bounds__(kBlock) restrict_tile_kernel(RArgs a, VarPtrs vp) {
+  constexpr int FX = 2 * RTX + N - 2, FY = 2 * RTY + N - 2, FZ = 2 * RTZ + N - 2;
+  __shared__ double fin[FZ * FY * FX];
+  __shared__ double ez[RTZ * FY * FX];
+  __shared__ double ey[RTZ * RTY * FX];
+  const int jb = find_job(a.job, a.njobs);
+  const RJob &J = a.job[jb];
+  const long b = (long)blockIdx.x - J.blk0;
+  const int tx0 = (int)(b % J.ntx) * RTX, ty0 = (int)((b / J.ntx) % J.nty) * RTY,
+            tz0 = (int)(b / ((long)J.ntx * J.nty)) * RTZ;
+  const int vx = min(RTX, J.ncx - tx0), vy = min(RTY, J.ncy - ty0), vz = min(RTZ, J.ncz - tz0);
+  // fine points the valid coarse points of the tile read (inside the fine box)
+  const int ex = 2 * (vx - 1) + N, eyy = 2 * (vy - 1) + N, ezz = 2 * (vz - 1) + N;
+  const int tid = threadIdx.x;
+  const int v = blockIdx.y;
+  const double *__restrict__ src = vp.src[v] + J.foff + (J.ifb[0] + 2 * tx0) + (long)(J.ifb[2] + 2 * ty0) * J.fm +
+                                   (long)(J.ifb[4] + 2 * tz0) * J.fmn;
+  for (int idx = tid; idx < FX * FY * FZ; idx += kBlock) {
+    const int x = idx % FX, y = (idx / FX) % FY, z = idx / (FX * FY);
+    fin[idx] = (x < ex && y < eyy && z < ezz) ? src[x + (long)y * J.fm + (long)z * J.fmn] : 0.0;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < RTZ * FY * FX; idx += kBlock) {  // z sums
+    const int x = idx % FX, y = (idx / FX) % FY, cz = idx / (FX * FY);
+    double e = 0;
+#pragma unroll
+    for (int p = 0; p < N; p++) e += a.c[p] * fin[x + FX * (y + FY * (2 * cz + p))];
+    ez[idx] = e;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < RTZ * RTY * FX; idx += kBlock) {  // y sums
+    const int x = idx % FX, cy = (idx / FX) % RTY, cz = idx / (FX * RTY);
+    double dv = 0;
+#pragma unroll
+    for (int by = 0; by < N; by++) dv += a.c[by] * ez[x + FX * ((2 * cy + by) + FY * cz)];
+    ey[idx] = dv;
+  }
+  __syncthreads();
+  const int cx = tid % RTX, cy = (tid / RTX) % RTY, cz = tid / (RTX * RTY);  // x sums, one coarse point each
+  if (cx < vx && cy < vy && cz < vz) {
+    double sum = 0;
+#pragma unroll
+    for (int ax = 0; ax < N; ax++) sum += a.c[ax] * ey[(2 * cx + ax) + FX * (cy + RTY * cz)];
+    vp.dst[v][J.coff + (J.icb[0] + tx0 + cx) + (long)(J.icb[2] + ty0 + cy) * J.cm +
+              (long)(J.icb[4] + tz0 + cz) * J.cmn] = sum;
+  }
+}
+static_assert(RTX * RTY * RTZ == kBlock, "one coarse point per thread in the x sweep");
 
 // flagprolong of one fine box (flag.c:143-206; no symmetry halves, no
 // process ghosts): depth from the nearest face if at most nbuffer, else 0
@@ -380,6 +440,7 @@ long assign_blocks(Job *dst, const Job *src, int nj, long (*units)(const Job &))
 
 long prolong_units(const PJob &J) { return J.noct; }
 long restrict_units(const RJob &J) { return J.npts; }
+long restrict_tiles(const RJob &J) { return (long)J.ntx * J.nty * J.ntz * kBlock; }
 
 int launch_prolong(PArgs &args, const std::vector<PJob> &jobs, double *const *dst, double *const *src, int n,
                    int order) {
@@ -409,12 +470,14 @@ int launch_prolong(PArgs &args, const std::vector<PJob> &jobs, double *const *ds
   return BAMHIP_OK;
 }
 
+int g_restrict_lds = 1;  // restriction through LDS tiles (restrict_tile_kernel)
+
 int launch_restrict(RArgs &args, const std::vector<RJob> &jobs, double *const *dst, double *const *src, int n,
                     int order) {
   for (size_t j0 = 0; j0 < jobs.size(); j0 += kMaxJobs) {
     const int nj = (int)std::min<size_t>(kMaxJobs, jobs.size() - j0);
     args.njobs = nj;
-    const long blk = assign_blocks(args.job, jobs.data() + j0, nj, restrict_units);
+    const long blk = assign_blocks(args.job, jobs.data() + j0, nj, g_restrict_lds ? restrict_tiles : restrict_units);
     if (blk == 0) continue;
     for (int v0 = 0; v0 < n; v0 += kMaxVars) {
       const int nv = std::min(kMaxVars, n - v0);
@@ -425,11 +488,20 @@ int launch_restrict(RArgs &args, const std::vector<RJob> &jobs, double *const *d
       }
       const dim3 grid((unsigned)blk, (unsigned)nv);
       hipStream_t st = stream();
-      switch (order) {
-        case 2: restrict_kernel<2><<<grid, kBlock, 0, st>>>(args, vp); break;
-        case 4: restrict_kernel<4><<<grid, kBlock, 0, st>>>(args, vp); break;
-        case 6: restrict_kernel<6><<<grid, kBlock, 0, st>>>(args, vp); break;
-        default: restrict_kernel<8><<<grid, kBlock, 0, st>>>(args, vp); break;
+      if (g_restrict_lds) {
+        switch (order) {
+          case 2: restrict_tile_kernel<2><<<grid, kBlock, 0, st>>>(args, vp); break;
+          case 4: restrict_tile_kernel<4><<<grid, kBlock, 0, st>>>(args, vp); break;
+          case 6: restrict_tile_kernel<6><<<grid, kBlock, 0, st>>>(args, vp); break;
+          default: restrict_tile_kernel<8><<<grid, kBlock, 0, st>>>(args, vp); break;
+        }
+      } else {
+        switch (order) {
+          case 2: restrict_kernel<2><<<grid, kBlock, 0, st>>>(args, vp); break;
+          case 4: restrict_kernel<4><<<grid, kBlock, 0, st>>>(args, vp); break;
+          case 6: restrict_kernel<6><<<grid, kBlock, 0, st>>>(args, vp); break;
+          default: restrict_kernel<8><<<grid, kBlock, 0, st>>>(args, vp); break;
+        }
       }
       BH_HIP(hipGetLastError());
     }
@@ -667,7 +739,11 @@ int bamhip_restrict_varlist(bamhip_level *lf, bamhip_level *lc, int n, const int
     if (2 * (J.ncx - 1) + order - 1 > J.ifb[1] - J.ifb[0] || 2 * (J.ncy - 1) + order - 1 > J.ifb[3] - J.ifb[2] ||
         2 * (ncz - 1) + order - 1 > J.ifb[5] - J.ifb[4])
       return set_error(BAMHIP_EINVAL, "%s: fine box %d: region not aligned with its parent", what, b);
+    J.ncz = ncz;
     J.npts = (long)J.ncx * J.ncy * ncz;
+    J.ntx = (J.ncx + RTX - 1) / RTX;
+    J.nty = (J.ncy + RTY - 1) / RTY;
+    J.ntz = (ncz + RTZ - 1) / RTZ;
     jobs.push_back(J);
   }
   RArgs args{};

@@ -1000,6 +1000,19 @@ __global__ void __launch_bounds__(NTHR, OCC) z4c_rhs_tile(Z4cPtrs P, TileJobs J,
   TP<L> T{tile + toff};
   const double *const *u = P.u;
 
+  // the first fields' transfers go out before the point values are loaded,
+  // so that a block waits for one round trip to HBM at its start, not two
+  using St = typename std::conditional<NB == 0, Stager<L>, DmaStager<L, (NB == 0 ? 2 : NB), VEC>>::type;
+  St st(G, i0, j0, k0, tid);
+  constexpr int NBUF = NB == 0 ? 1 : NB;
+  constexpr int BUFD = StageBuf<L, NB, VEC>::n / NBUF;
+  if constexpr (NB == 0) {
+    st.load(u[STAGE_ORDER[0]]);
+  } else {
+#pragma unroll
+    for (int p = 0; p < NB - 1; p++) st.issue(u[STAGE_ORDER[p]], tile + p * BUFD);
+  }
+
   // point values; threads outside the box use flat data so that their
   // (discarded) arithmetic stays finite
   double g[6], A[6], Gm[3], beta[3], Bv[3];
@@ -1025,16 +1038,6 @@ __global__ void __launch_bounds__(NTHR, OCC) z4c_rhs_tile(Z4cPtrs P, TileJobs J,
     Khat = u[13][q];
     Theta = u[17][q];
     alpha = u[18][q];
-  }
-  using St = typename std::conditional<NB == 0, Stager<L>, DmaStager<L, (NB == 0 ? 2 : NB), VEC>>::type;
-  St st(G, i0, j0, k0, tid);
-  constexpr int NBUF = NB == 0 ? 1 : NB;
-  constexpr int BUFD = StageBuf<L, NB, VEC>::n / NBUF;
-  if constexpr (NB == 0) {
-    st.load(u[STAGE_ORDER[0]]);
-  } else {
-#pragma unroll
-    for (int p = 0; p < NB - 1; p++) st.issue(u[STAGE_ORDER[p]], tile + p * BUFD);
   }
   // stage s: make field s visible in LDS, start the transfer of a later field
   auto advance = [&](int sidx) {
