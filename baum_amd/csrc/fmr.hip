@@ -41,7 +41,8 @@ struct PJob {
   long foff, coff;          // box offsets in the level arrays
   int fm, fmn, cm, cmn;     // row and plane strides of fine and coarse box
   int ifb[6], ifb0[6], icb[6];
-  int nox, noy;             // octets per row and per column
+  int nox, noy, noz;        // octets per row, column and plane
+  int ntx, nty, ntz;        // octet tiles of the LDS kernel per direction
   long noct;                // octets in the bounding box
   long blk0;                // first block of this box
 };
@@ -146,6 +147,102 @@ __global__ void __launch_bounds__(kBlock) prolong_kernel(PArgs a, VarPtrs vp) {
       fv[qf] = o[r][s][0];
       fv[qf + 1] = o[r][s][1];
     }
+}
+
+// Prolongation of a tile of PTX x PTY x PTZ octets through LDS: the coarse
+// footprint is staged once, then the z sums of every coarse (x, y) column for
+// both children in z, the y sums for both children in y and the x sums, as
+// the reference's three sweeps (:652-672, 712-731, 770-794).  Every fine
+// value is the same sequence of products and sums as in prolong_kernel (a
+// left child sums terms 0..N-1 of its window with c1, a right child terms
+// 1..N with c2, each from 0 in increasing order), so the results are bitwise
+// equal; the coarse data is read once per tile instead of (N+1)^3 / 8 times
+// per fine point.
+constexpr int PTX = 8, PTY = 4, PTZ = 4;
+template <int N>
+__global__ void __launch_bounds__(kBlock) prolong_tile_kernel(PArgs a, VarPtrs vp) {
+  constexpr int CX = PTX + N, CY = PTY + N, CZ = PTZ + N;  // coarse footprint
+  __shared__ double cin[CZ * CY * CX];
+  __shared__ double dz[2 * PTZ * CY * CX];   // [fine z][coarse y][coarse x]
+  __shared__ double dy[2 * PTZ * 2 * PTY * CX];  // [fine z][fine y][coarse x]
+  const int jb = find_job(a.job, a.njobs);
+  const PJob &J = a.job[jb];
+  const long b = (long)blockIdx.x - J.blk0;
+  const int ox0 = (int)(b % J.ntx) * PTX, oy0 = (int)((b / J.ntx) % J.nty) * PTY,
+            oz0 = (int)(b / ((long)J.ntx * J.nty)) * PTZ;
+  const int vx = min(PTX, J.nox - ox0), vy = min(PTY, J.noy - oy0), vz = min(PTZ, J.noz - oz0);
+  const int tid = threadIdx.x;
+  // does any fine point of the tile get a value?  (fbox0 skip, :845-850)
+  int mine = 0;
+  for (int idx = tid; idx < 8 * PTX * PTY * PTZ; idx += kBlock) {
+    const int fx = idx % (2 * PTX), fy = (idx / (2 * PTX)) % (2 * PTY), fz = idx / (4 * PTX * PTY);
+    const int ox = fx / 2, oy = fy / 2, oz = fz / 2;
+    if (ox >= vx || oy >= vy || oz >= vz) continue;
+    const int fi = J.ifb[0] + 2 * (ox0 + ox), j = J.ifb[2] + 2 * oy0 + fy, k = J.ifb[4] + 2 * oz0 + fz;
+    const bool valid = j <= J.ifb[3] && k <= J.ifb[5];
+    const bool skip = fi >= J.ifb0[0] && fi <= J.ifb0[1] && j >= J.ifb0[2] && j <= J.ifb0[3] && k >= J.ifb0[4] &&
+                      k <= J.ifb0[5];
+    mine |= (valid && !skip);
+  }
+  if (!__syncthreads_or(mine)) return;
+  const int v = blockIdx.y;
+  const double *__restrict__ cv =
+      vp.src[v] + J.coff + (J.icb[0] + ox0) + (long)(J.icb[2] + oy0) * J.cm + (long)(J.icb[4] + oz0) * J.cmn;
+  const int ex = vx + N, ey = vy + N, ez = vz + N;  // coarse points the valid octets read
+  for (int idx = tid; idx < CZ * CY * CX; idx += kBlock) {
+    const int x = idx % CX, y = (idx / CX) % CY, z = idx / (CX * CY);
+    cin[idx] = (x < ex && y < ey && z < ez) ? cv[x + (long)y * J.cm + (long)z * J.cmn] : 0.0;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < 2 * PTZ * CY * CX; idx += kBlock) {  // z sums
+    const int x = idx % CX, y = (idx / CX) % CY, fz = idx / (CX * CY);
+    const int oz = fz / 2, sz = fz % 2;
+    double d = 0;
+    if (sz == 0) {
+#pragma unroll
+      for (int p = 0; p < N; p++) d += a.c1[p] * cin[x + CX * (y + CY * (oz + p))];
+    } else {
+#pragma unroll
+      for (int p = 0; p < N; p++) d += a.c2[p] * cin[x + CX * (y + CY * (oz + p + 1))];
+    }
+    dz[idx] = d;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < 4 * PTZ * PTY * CX; idx += kBlock) {  // y sums
+    const int x = idx % CX, fy = (idx / CX) % (2 * PTY), fz = idx / (CX * 2 * PTY);
+    const int oy = fy / 2, r = fy % 2;
+    double e = 0;
+    if (r == 0) {
+#pragma unroll
+      for (int by = 0; by < N; by++) e += a.c1[by] * dz[x + CX * ((oy + by) + CY * fz)];
+    } else {
+#pragma unroll
+      for (int by = 1; by <= N; by++) e += a.c2[by - 1] * dz[x + CX * ((oy + by) + CY * fz)];
+    }
+    dy[idx] = e;
+  }
+  __syncthreads();
+  double *__restrict__ fv = vp.dst[v] + J.foff;
+  for (int idx = tid; idx < 8 * PTX * PTY * PTZ; idx += kBlock) {  // x sums and stores
+    const int fx = idx % (2 * PTX), fy = (idx / (2 * PTX)) % (2 * PTY), fz = idx / (4 * PTX * PTY);
+    const int ox = fx / 2, oy = fy / 2, oz = fz / 2, ac = fx % 2;
+    if (ox >= vx || oy >= vy || oz >= vz) continue;
+    const int fi = J.ifb[0] + 2 * (ox0 + ox), j = J.ifb[2] + 2 * oy0 + fy, k = J.ifb[4] + 2 * oz0 + fz;
+    const bool valid = j <= J.ifb[3] && k <= J.ifb[5];
+    const bool skip = fi >= J.ifb0[0] && fi <= J.ifb0[1] && j >= J.ifb0[2] && j <= J.ifb0[3] && k >= J.ifb0[4] &&
+                      k <= J.ifb0[5];
+    if (!valid || skip) continue;
+    const double *row = dy + CX * (fy + 2 * PTY * fz);
+    double o = 0;
+    if (ac == 0) {
+#pragma unroll
+      for (int ax = 0; ax < N; ax++) o += a.c1[ax] * row[ox + ax];
+    } else {
+#pragma unroll
+      for (int ax = 1; ax <= N; ax++) o += a.c2[ax - 1] * row[ox + ax];
+    }
+    fv[fi + ac + (long)j * J.fm + (long)k * J.fmn] = o;
+  }
 }
 
 // Restriction of one coarse point from the N^3 fine points starting at
@@ -439,6 +536,8 @@ long assign_blocks(Job *dst, const Job *src, int nj, long (*units)(const Job &))
 }
 
 long prolong_units(const PJob &J) { return J.noct; }
+long prolong_tiles(const PJob &J) { return (long)J.ntx * J.nty * J.ntz * kBlock; }
+int g_prolong_lds = 1;  // prolongation through LDS tiles (prolong_tile_kernel)
 long restrict_units(const RJob &J) { return J.npts; }
 long restrict_tiles(const RJob &J) { return (long)J.ntx * J.nty * J.ntz * kBlock; }
 
@@ -447,7 +546,7 @@ int launch_prolong(PArgs &args, const std::vector<PJob> &jobs, double *const *ds
   for (size_t j0 = 0; j0 < jobs.size(); j0 += kMaxJobs) {
     const int nj = (int)std::min<size_t>(kMaxJobs, jobs.size() - j0);
     args.njobs = nj;
-    const long blk = assign_blocks(args.job, jobs.data() + j0, nj, prolong_units);
+    const long blk = assign_blocks(args.job, jobs.data() + j0, nj, g_prolong_lds ? prolong_tiles : prolong_units);
     if (blk == 0) continue;
     for (int v0 = 0; v0 < n; v0 += kMaxVars) {
       const int nv = std::min(kMaxVars, n - v0);
@@ -458,11 +557,20 @@ int launch_prolong(PArgs &args, const std::vector<PJob> &jobs, double *const *ds
       }
       const dim3 grid((unsigned)blk, (unsigned)nv);
       hipStream_t st = stream();
-      switch (order) {
-        case 2: prolong_kernel<2><<<grid, kBlock, 0, st>>>(args, vp); break;
-        case 4: prolong_kernel<4><<<grid, kBlock, 0, st>>>(args, vp); break;
-        case 6: prolong_kernel<6><<<grid, kBlock, 0, st>>>(args, vp); break;
-        default: prolong_kernel<8><<<grid, kBlock, 0, st>>>(args, vp); break;
+      if (g_prolong_lds) {
+        switch (order) {
+          case 2: prolong_tile_kernel<2><<<grid, kBlock, 0, st>>>(args, vp); break;
+          case 4: prolong_tile_kernel<4><<<grid, kBlock, 0, st>>>(args, vp); break;
+          case 6: prolong_tile_kernel<6><<<grid, kBlock, 0, st>>>(args, vp); break;
+          default: prolong_tile_kernel<8><<<grid, kBlock, 0, st>>>(args, vp); break;
+        }
+      } else {
+        switch (order) {
+          case 2: prolong_kernel<2><<<grid, kBlock, 0, st>>>(args, vp); break;
+          case 4: prolong_kernel<4><<<grid, kBlock, 0, st>>>(args, vp); break;
+          case 6: prolong_kernel<6><<<grid, kBlock, 0, st>>>(args, vp); break;
+          default: prolong_kernel<8><<<grid, kBlock, 0, st>>>(args, vp); break;
+        }
       }
       BH_HIP(hipGetLastError());
     }
@@ -685,7 +793,11 @@ int bamhip_prolong_varlist(bamhip_level *lc, bamhip_level *lf, int n, const int 
       return set_error(BAMHIP_EINVAL, "%s: fine box %d: region not aligned with its parent", what, b);
     J.nox = ex / 2;
     J.noy = (ey + 1) / 2;
-    J.noct = (long)J.nox * J.noy * ((ez + 1) / 2);
+    J.noz = (ez + 1) / 2;
+    J.noct = (long)J.nox * J.noy * J.noz;
+    J.ntx = (J.nox + PTX - 1) / PTX;
+    J.nty = (J.noy + PTY - 1) / PTY;
+    J.ntz = (J.noz + PTZ - 1) / PTZ;
     jobs.push_back(J);
   }
   PArgs args{};
