@@ -106,6 +106,8 @@ def lib():
     _sig(L, "bamhip_vladd", i, [vp, i, IP, d, IP, d, IP])
     _sig(L, "bamhip_rk4_stage_update", i, [vp, i, IP, IP, IP, IP, d, d])
     _sig(L, "bamhip_z4c_rk4_step", i, [vp, C.POINTER(Z4cParams), d, IP, IP, IP, IP, i, i])
+    _sig(L, "bamhip_z4c_rk4_stages", i, [vp, C.POINTER(Z4cParams), d, IP, IP, IP, IP, i])
+    _sig(L, "bamhip_z4c_rk4_step_bo", i, [vp, C.POINTER(Z4cParams), d, IP, IP, IP, IP, i, IP, i, i, i])
     _sig(L, "bamhip_z4c_algcon", i, [vp, IP, IP, i, i])
     _sig(L, "bamhip_z4c_ham_l2", i, [vp, IP, d, i, DP])
     _sig(L, "bamhip_z4c_constraints_l2", i, [vp, IP, d, i, IP, DP])
@@ -290,6 +292,16 @@ class Level:
     def z4c_rk4_step(self, params, dt, uc, up, r, v, pgh=0, algcon=True):
         check(lib().bamhip_z4c_rk4_step(self.h, C.byref(params), dt, _iarr(uc), _iarr(up), _iarr(r), _iarr(v),
                                         pgh, 1 if algcon else 0), "bamhip_z4c_rk4_step")
+
+    def z4c_rk4_stages(self, params, dt, uc, up, r, v, pgh=0):
+        check(lib().bamhip_z4c_rk4_stages(self.h, C.byref(params), dt, _iarr(uc), _iarr(up), _iarr(r), _iarr(v), pgh),
+              "bamhip_z4c_rk4_stages")
+
+    def z4c_rk4_step_bo(self, params, dt, uc, up, r, v, upp, timeinterpolation, iteration, pgh=0, algcon=True):
+        """RK4 step with the Berger-Oliger list operations around it (upp=None: finest level)"""
+        check(lib().bamhip_z4c_rk4_step_bo(self.h, C.byref(params), dt, _iarr(uc), _iarr(up), _iarr(r), _iarr(v), pgh,
+                                           _iarr(upp) if upp is not None else None, timeinterpolation, iteration,
+                                           1 if algcon else 0), "bamhip_z4c_rk4_step_bo")
 
     def z4c_algcon(self, g, A, subtractA=1, normalizedetg=1):
         check(lib().bamhip_z4c_algcon(self.h, _iarr(g), _iarr(A), subtractA, normalizedetg), "bamhip_z4c_algcon")

@@ -263,6 +263,30 @@ __global__ void periodic_all_kernel(Ptrs25 u, DevBox B, int g, int px, int py, i
 }
 
 // ---------------------------------------------------------------- algcon
+// z4_algcon.m:14-48 at one point
+__device__ __forceinline__ void algcon_point(double g[6], double A[6], int subtractA, int normalizedetg) {
+  auto det = [&]() {
+    return g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) + g[2] * (g[1] * g[4] - g[3] * g[2]);
+  };
+  double detg = det();
+  if (detg <= 0.0) detg = 1.0;
+  if (normalizedetg) {
+    const double e = detg - 1.0;
+    const double aux = (fabs(e) < 1e-12) ? 1.0 - e / 3.0 : pow(detg, -1.0 / 3.0);
+#pragma unroll
+    for (int c = 0; c < 6; c++) g[c] *= aux;
+  }
+  detg = det();
+  if (detg <= 0.0) detg = 1.0;
+  const double a0 = g[3] * g[5] - g[4] * g[4], a1 = g[2] * g[4] - g[1] * g[5], a2 = g[1] * g[4] - g[2] * g[3];
+  const double a3 = g[0] * g[5] - g[2] * g[2], a4 = g[1] * g[2] - g[0] * g[4], a5 = g[0] * g[3] - g[1] * g[1];
+  const double trA = (a0 * A[0] + a3 * A[3] + a5 * A[5] + 2.0 * (a1 * A[1] + a2 * A[2] + a4 * A[4])) / detg;
+  if (subtractA) {
+#pragma unroll
+    for (int c = 0; c < 6; c++) A[c] -= trA / 3.0 * g[c];
+  }
+}
+
 // z4_algcon.m:14-48 on all points
 __global__ void algcon_kernel(Ptrs25 g6, Ptrs25 A6, long n, int subtractA, int normalizedetg) {
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
@@ -272,32 +296,84 @@ __global__ void algcon_kernel(Ptrs25 g6, Ptrs25 A6, long n, int subtractA, int n
       g[c] = g6.p[c][q];
       A[c] = A6.p[c][q];
     }
-    auto det = [&]() {
-      return g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
-             g[2] * (g[1] * g[4] - g[3] * g[2]);
-    };
-    double detg = det();
-    if (detg <= 0.0) detg = 1.0;
-    if (normalizedetg) {
-      const double e = detg - 1.0;
-      const double aux = (fabs(e) < 1e-12) ? 1.0 - e / 3.0 : pow(detg, -1.0 / 3.0);
-#pragma unroll
-      for (int c = 0; c < 6; c++) g[c] *= aux;
-    }
-    detg = det();
-    if (detg <= 0.0) detg = 1.0;
-    const double a0 = g[3] * g[5] - g[4] * g[4], a1 = g[2] * g[4] - g[1] * g[5], a2 = g[1] * g[4] - g[2] * g[3];
-    const double a3 = g[0] * g[5] - g[2] * g[2], a4 = g[1] * g[2] - g[0] * g[4], a5 = g[0] * g[3] - g[1] * g[1];
-    const double trA = (a0 * A[0] + a3 * A[3] + a5 * A[5] + 2.0 * (a1 * A[1] + a2 * A[2] + a4 * A[4])) / detg;
-    if (subtractA) {
-#pragma unroll
-      for (int c = 0; c < 6; c++) A[c] -= trA / 3.0 * g[c];
-    }
+    algcon_point(g, A, subtractA, normalizedetg);
 #pragma unroll
     for (int c = 0; c < 6; c++) {
       g6.p[c][q] = g[c];
       A6.p[c][q] = A[c];
     }
+  }
+}
+
+// First RK4 stage update of a step with the list operations before it folded
+// in (evolve.c:298-340, rk.c:225-240): with bo, upp = ba*uc + bb*up (the
+// time-interpolation begin, orders 2 and 3, as bamhip_vladd forms it); then
+// up = uc; uc += wc*r; v = up + wv*r.  Each value is the same expression as
+// in the separate vladd and rk4_update passes, so the results are bitwise
+// theirs.
+__global__ void rk4_first_kernel(Ptrs25 uc, Ptrs25 up, Ptrs25 r, Ptrs25 v, Ptrs25 upp, double wc, double wv, int bo,
+                                 double ba, double bb, long n) {
+  const int k = blockIdx.y;
+  double *__restrict__ c = uc.p[k];
+  double *__restrict__ p = up.p[k];
+  const double *__restrict__ rr = r.p[k];
+  double *__restrict__ vv = v.p[k];
+  double *__restrict__ pp = upp.p[k];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double ci = c[i];
+    if (bo) {
+      double s = 0.0;
+      if (ba != 0.0) s += ba * ci;
+      if (bb != 0.0) s += bb * p[i];
+      pp[i] = s;
+    }
+    p[i] = ci;
+    const double ri = rr[i];
+    double cn = ci;
+    cn += wc * ri;
+    c[i] = cn;
+    vv[i] = ci + wv * ri;
+  }
+}
+
+// Last RK4 stage update of a step with the list operations after it folded
+// in (rk.c:250-252, evolve.c:367-380, z4.c:49-71): uc += wc*r; with bo,
+// upp += bc*uc (the time-interpolation end, as bamhip_vladd forms it); with
+// algcon, z4_algcon on the new point.  One thread per point, all 25 fields.
+__global__ void rk4_last_kernel(Ptrs25 uc, Ptrs25 r, Ptrs25 upp, double wc, int bo, double bc, int algcon,
+                                int subtractA, int normalizedetg, long n) {
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    double u[25];
+#pragma unroll
+    for (int k = 0; k < 25; k++) {
+      u[k] = uc.p[k][q];
+      u[k] += wc * r.p[k][q];
+    }
+    if (bo) {
+#pragma unroll
+      for (int k = 0; k < 25; k++) {
+        double s = 0.0;
+        s += 1.0 * upp.p[k][q];
+        s += bc * u[k];
+        upp.p[k][q] = s;
+      }
+    }
+    if (algcon) {
+      double g[6], A[6];
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        g[c] = u[c];
+        A[c] = u[7 + c];
+      }
+      algcon_point(g, A, subtractA, normalizedetg);
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        u[c] = g[c];
+        u[7 + c] = A[c];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 25; k++) uc.p[k][q] = u[k];
   }
 }
 
@@ -624,6 +700,46 @@ int bamhip_rk4_stage_update(bamhip_level *lev, int n, const int *uc, const int *
   }
   TimerScope t("rk4_update");
   rk4_update_kernel<<<stream_grid(lev->npoints, n), 256, 0, stream()>>>(C, P, R, V, wc, wv, lev->npoints);
+  BH_HIP(hipGetLastError());
+  return BAMHIP_OK;
+}
+
+int bamhip_rk4_first_update(bamhip_level *lev, const int *uc, const int *up, const int *r, const int *v, double wc,
+                            double wv, const int *upp, double ba, double bb) {
+  Ptrs25 C, P, R, V, PP;
+  int e;
+  if (!lev) return set_error(BAMHIP_EINVAL, "bamhip_rk4_first_update: NULL level");
+  if ((e = get25(lev, uc, 25, C, "rk4_first_update(uc)"))) return e;
+  if ((e = get25(lev, up, 25, P, "rk4_first_update(up)"))) return e;
+  if ((e = get25(lev, r, 25, R, "rk4_first_update(r)"))) return e;
+  if ((e = get25(lev, v, 25, V, "rk4_first_update(v)"))) return e;
+  if (upp) {
+    if ((e = get25(lev, upp, 25, PP, "rk4_first_update(upp)"))) return e;
+  } else {
+    PP = P;
+  }
+  TimerScope t("rk4_update");
+  rk4_first_kernel<<<stream_grid(lev->npoints, 25), 256, 0, stream()>>>(C, P, R, V, PP, wc, wv, upp != nullptr, ba, bb,
+                                                                         lev->npoints);
+  BH_HIP(hipGetLastError());
+  return BAMHIP_OK;
+}
+
+int bamhip_rk4_last_update(bamhip_level *lev, const int *uc, const int *r, double wc, const int *upp, double bc,
+                           int algcon, int subtractA, int normalizedetg) {
+  Ptrs25 C, R, PP;
+  int e;
+  if (!lev) return set_error(BAMHIP_EINVAL, "bamhip_rk4_last_update: NULL level");
+  if ((e = get25(lev, uc, 25, C, "rk4_last_update(uc)"))) return e;
+  if ((e = get25(lev, r, 25, R, "rk4_last_update(r)"))) return e;
+  if (upp) {
+    if ((e = get25(lev, upp, 25, PP, "rk4_last_update(upp)"))) return e;
+  } else {
+    PP = C;
+  }
+  TimerScope t("rk4_update");
+  rk4_last_kernel<<<stream_grid(lev->npoints, 1).x, 256, 0, stream()>>>(C, R, PP, wc, upp != nullptr, bc, algcon,
+                                                                         subtractA, normalizedetg, lev->npoints);
   BH_HIP(hipGetLastError());
   return BAMHIP_OK;
 }

@@ -60,6 +60,48 @@ int bamhip_z4c_rk4_stages(bamhip_level *lev, const bamhip_z4c_params *par, doubl
   return BAMHIP_OK;
 }
 
+/* One RK4 step of a level together with the list operations around it in
+ * the Berger-Oliger evolution (evolve.c:298-380, z4.c:49-71): the time
+ * interpolation begin of order o into upp (upp may be NULL), the four
+ * stages, the time interpolation end and algcon.  When the stages run
+ * separately from the RHS (small levels, see bamhip_z4c_set_rk_fusion), the
+ * copy u_p = u_c and the interpolation begin are folded into the first stage
+ * update, the interpolation end and algcon into the last; elsewhere the
+ * operations run one by one.  Bitwise the same either way. */
+int bamhip_z4c_rk4_step_bo(bamhip_level *lev, const bamhip_z4c_params *par, double dt, const int *uc, const int *up,
+                           const int *r, const int *v, int pgh, const int *upp, int timeinterpolation, int iteration,
+                           int algcon)
+{
+  static const double cend[4] = {0, 0, 1.0 / 2, 3. / 8};
+  const double wc[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
+  const double wv[4] = {dt / 2, dt / 2, dt, 0.0};
+  const int o = (iteration + 2 > timeinterpolation) ? timeinterpolation : iteration + 2;
+  long npoints = 0;
+  int e, s;
+  if (upp && (o < 2 || o > 3)) return BAMHIP_EINVAL;
+  if ((e = bamhip_level_npoints(lev, &npoints))) return e;
+  if (g_fused == 1 || (g_fused == 2 && npoints >= BAMHIP_FUSE_MIN_POINTS)) {
+    if (upp && (e = bamhip_bo_interp_begin(lev, timeinterpolation, iteration, BAMHIP_Z4C_NVARS, upp, uc, up, 0, 0, 0)))
+      return e;
+    if ((e = bamhip_z4c_rk4_stages(lev, par, dt, uc, up, r, v, pgh))) return e;
+    if (upp && (e = bamhip_bo_interp_end(lev, timeinterpolation, iteration, BAMHIP_Z4C_NVARS, upp, uc))) return e;
+    return algcon ? bamhip_z4c_algcon_list(lev, par, uc) : BAMHIP_OK;
+  }
+  for (s = 0; s < 4; s++) {
+    const int *src = (s == 0) ? uc : v;
+    if ((e = bamhip_z4c_rhs(lev, par, r, 0, 0.0, src))) return e;
+    if ((e = bamhip_z4c_set_boundary(lev, par, r, 0, 0.0, src, pgh))) return e;
+    if (s == 0)
+      e = bamhip_rk4_first_update(lev, uc, up, r, v, wc[0], wv[0], upp, o == 2 ? 0.5 : 0.75, o == 2 ? 0.0 : -0.125);
+    else if (s < 3)
+      e = bamhip_rk4_stage_update(lev, BAMHIP_Z4C_NVARS, uc, up, r, v, wc[s], wv[s]);
+    else
+      e = bamhip_rk4_last_update(lev, uc, r, wc[3], upp, cend[o], algcon, par->subtractA, par->normalizedetg);
+    if (e) return e;
+  }
+  return BAMHIP_OK;
+}
+
 int bamhip_z4c_algcon_list(bamhip_level *lev, const bamhip_z4c_params *par, const int *uc)
 {
   int gi[6], ai[6], s;

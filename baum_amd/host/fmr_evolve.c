@@ -25,15 +25,9 @@ static int advance_level(const bamhip_fmr_hierarchy *H, int l)
   const double dt = H->dt0 / (double)(1L << l);
   const int it = H->iteration[l];
   int e;
-  if (l < H->lmax &&
-      (e = bamhip_bo_interp_begin(lev, H->timeinterpolation, it, BAMHIP_Z4C_NVARS, H->upp, H->u, H->up, NULL, NULL,
-                                  NULL)))
+  if ((e = bamhip_z4c_rk4_step_bo(lev, &H->params[l], dt, H->u, H->up, H->r, H->v, l == 0 ? H->pgh : 0,
+                                   l < H->lmax ? H->upp : NULL, H->timeinterpolation, it, 1)))
     return e;
-  if ((e = bamhip_z4c_rk4_stages(lev, &H->params[l], dt, H->u, H->up, H->r, H->v, l == 0 ? H->pgh : 0))) return e;
-  if (l < H->lmax &&
-      (e = bamhip_bo_interp_end(lev, H->timeinterpolation, it, BAMHIP_Z4C_NVARS, H->upp, H->u)))
-    return e;
-  if ((e = bamhip_z4c_algcon_list(lev, &H->params[l], H->u))) return e;
   H->iteration[l] = it + 1;
   return BAMHIP_OK;
 }

@@ -371,6 +371,23 @@ int bamhip_z4c_rk4_stages(bamhip_level *lev, const bamhip_z4c_params *par, doubl
                           const int *r, const int *v, int pgh);
 int bamhip_z4c_algcon_list(bamhip_level *lev, const bamhip_z4c_params *par, const int *uc);
 
+/* One level step of the Berger-Oliger evolution (evolve.c:298-380 around
+ * evolve_rk_rk4, then z4_algcon_wrapper, z4.c:49-71): time interpolation
+ * begin into upp (NULL: finest level, none), the RK4 stages, interpolation
+ * end and (algcon != 0) algcon, with orders 2 and 3 (o = min(
+ * timeinterpolation, iteration + 2)).  With separate stage updates the list
+ * operations are folded into the first and last update passes
+ * (bamhip_rk4_first_update: upp = ba*uc + bb*up, up = uc, uc += wc*r,
+ * v = up + wv*r; bamhip_rk4_last_update: uc += wc*r, upp += bc*uc, algcon);
+ * the results are bitwise those of the separate calls. */
+int bamhip_z4c_rk4_step_bo(bamhip_level *lev, const bamhip_z4c_params *par, double dt, const int *uc, const int *up,
+                           const int *r, const int *v, int pgh, const int *upp, int timeinterpolation, int iteration,
+                           int algcon);
+int bamhip_rk4_first_update(bamhip_level *lev, const int *uc, const int *up, const int *r, const int *v, double wc,
+                            double wv, const int *upp, double ba, double bb);
+int bamhip_rk4_last_update(bamhip_level *lev, const int *uc, const int *r, double wc, const int *upp, double bc,
+                           int algcon, int subtractA, int normalizedetg);
+
 /* One RK4 stage of evolve_rk_rk4 (rk.c:231-252: evolve_rhs, set_boundary,
  * vladdto, vladd) with the update fused into the RHS kernels: at every point
  * uc += wc * RHS(src) and, when vout != NULL, vout = up + wv * RHS(src); the

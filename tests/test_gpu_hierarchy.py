@@ -141,3 +141,33 @@ def test_graph_replay_matches_eager():
     finally:
         B.Hierarchy.free_graph(g)
     assert HA.nonfinite() == 0
+
+
+@pytest.mark.parametrize("iteration", [0, 1])
+def test_folded_level_step_bitwise_equal_separate(iteration):
+    """an RK4 level step with the time-interpolation begin / end, the u_p copy
+    and algcon folded into the first and last update passes equals the
+    separate vladd / stage / algcon sequence bitwise"""
+    dt0 = 0.125
+    res = []
+    for folded in (False, True):
+        H, levels = _gpu_hierarchy(dt0)
+        lv = levels[1]
+        par = B.z4c_params()
+        for _ in range(2):      # a nontrivial u_p and u_pp
+            lv.z4c_rk4_step(par, dt0 / 2, U, UP, R, V, pgh=0)
+        lv.bo_interp_begin(TI, 0, UPP, U, UP)
+        if folded:
+            lv.z4c_rk4_step_bo(par, dt0 / 2, U, UP, R, V, UPP, TI, iteration)
+        else:
+            B.set_rk_fusion(0)
+            lv.bo_interp_begin(TI, iteration, UPP, U, UP)
+            lv.z4c_rk4_stages(par, dt0 / 2, U, UP, R, V)
+            lv.bo_interp_end(TI, iteration, UPP, U)
+            lv.z4c_algcon(U[0:6], U[7:13])
+            B.set_rk_fusion(2)
+        res.append([lv.get_state(x) for x in (U, UP, UPP, V)])
+        for l in levels:
+            l.free()
+    for a, b in zip(*res):
+        np.testing.assert_array_equal(a, b)
