@@ -117,6 +117,7 @@ def lib():
     _sig(L, "bamhip_restrict_prolong_varlist", i, [vp, vp, i, IP, IP, i, i])
     _sig(L, "bamhip_bo_interp_begin", i, [vp, i, i, i, IP, IP, IP, IP, IP, IP])
     _sig(L, "bamhip_bo_interp_end", i, [vp, i, i, i, IP, IP])
+    _sig(L, "bamhip_puncture_track", i, [vp, IP, IP, d, i, i, vp])
     _sig(L, "bamhip_timer_enable", i, [i])
     _sig(L, "bamhip_timer_reset", i, [])
     _sig(L, "bamhip_timer_get", i, [C.c_char_p, DP, C.POINTER(C.c_long)])
@@ -302,6 +303,12 @@ class Level:
         check(lib().bamhip_z4c_rk4_step_bo(self.h, C.byref(params), dt, _iarr(uc), _iarr(up), _iarr(r), _iarr(v), pgh,
                                            _iarr(upp) if upp is not None else None, timeinterpolation, iteration,
                                            1 if algcon else 0), "bamhip_z4c_rk4_step_bo")
+
+    def puncture_track(self, beta, beta_p, dt, pos, order=4):
+        """one tracking step of the punctures whose coordinates are in the
+        device array pos (shape (np, 3)), updated in place"""
+        check(lib().bamhip_puncture_track(self.h, _iarr(beta), _iarr(beta_p), dt, order, pos.shape[0], pos.ptr),
+              "bamhip_puncture_track")
 
     def z4c_algcon(self, g, A, subtractA=1, normalizedetg=1):
         check(lib().bamhip_z4c_algcon(self.h, _iarr(g), _iarr(A), subtractA, normalizedetg), "bamhip_z4c_algcon")

@@ -408,6 +408,18 @@ int bamhip_z4c_bc_rk4_stage(bamhip_level *lev, const bamhip_z4c_params *par, con
                             const int *uc, const int *up, const int *vout, double wc, double wv, int pgh);
 int bamhip_z4c_set_rk_fusion(int mode);
 
+/* ---------------------------------------------------------- punctures */
+/* Replaces track_moving_puncture_integrate (src/projects/Gauge/
+ * moving_punctures_integrate.c:29-85) on the finest level of a puncture: three
+ * ICN iterations x = x_old - dt (beta(x) + beta_p(x_old)) / 2 with the shift
+ * interpolated as interpolate_xyz_scalar does (interpolate.c:419-470,
+ * Lagrange of `order` points, 0 where no box holds the cube).  beta, beta_p:
+ * 3 variable indices of the shift at the new and the previous time level;
+ * pos: device array of 3*np coordinates, updated in place (np <= 64), so the
+ * tracker can run inside a captured graph. */
+int bamhip_puncture_track(bamhip_level *lev, const int *beta, const int *beta_p, double dt, int order, int np,
+                          double *pos);
+
 /* --------------------------------------------------------------- graphs */
 /* Capture the device work issued on the library stream between begin and
  * end into an executable graph (hipGraph), replayed by launch.  Timers are

@@ -74,6 +74,12 @@ def lib():
         L.ob_z4c_rk4_step.restype = C.c_int
         L.ob_z4c_rk4_step.argtypes = [C.POINTER(Box), C.POINTER(Z4cParams), C.c_double, C.c_int,
                                       C.c_int, C.POINTER(DP), C.POINTER(DP), C.c_int]
+        L.ob_interp_point.restype = C.c_double
+        L.ob_interp_point.argtypes = [C.c_int, C.POINTER(Box), C.POINTER(DP), C.c_double, C.c_double, C.c_double,
+                                      C.c_int, C.POINTER(C.c_int)]
+        L.ob_track_puncture.restype = None
+        L.ob_track_puncture.argtypes = [C.c_int, C.POINTER(Box), C.POINTER(DP), C.POINTER(DP), C.c_double, C.c_int,
+                                        DP, DP]
         L.ob_z4c_ham_l2.restype = C.c_double
         L.ob_z4c_ham_l2.argtypes = [C.POINTER(Box), C.POINTER(DP), C.c_double, DP]
         L.ob_z4c_constraints_l2.restype = None
@@ -362,3 +368,29 @@ def c2p(P, cons, metric, pguess, rhoguess=None, method="hybrid"):
     fn(C.byref(P), n, cons.ctypes.data_as(DP), metric.ctypes.data_as(DP), prim.ctypes.data_as(DP),
        st.ctypes.data_as(IP), it.ctypes.data_as(IP))
     return prim, st, it, cons
+
+
+# ---------------------------------------------------------------- punctures
+def _box_ptrs(fields):
+    """double* array of per-box arrays (fields: list over components of lists over boxes)"""
+    flat = [np.ascontiguousarray(a, dtype=np.float64) for comp in fields for a in comp]
+    arr = (DP * len(flat))(*[a.ctypes.data_as(DP) for a in flat])
+    return arr, flat
+
+
+def interp_point(boxes, var, x, y, z, order=4):
+    """interpolate_xyz_scalar: var is a list over boxes; returns (value, found)"""
+    ptrs, keep = _box_ptrs([var])
+    f = C.c_int()
+    v = lib().ob_interp_point(len(boxes), _boxes(boxes), ptrs, x, y, z, order, C.byref(f))
+    return v, bool(f.value)
+
+
+def track_puncture(boxes, beta, beta_p, dt, pold, order=4):
+    """one step of track_moving_puncture_integrate; beta, beta_p: 3 lists over boxes"""
+    pb, kb = _box_ptrs(beta)
+    pp, kp = _box_ptrs(beta_p)
+    po = np.ascontiguousarray(pold, dtype=np.float64)
+    pn = np.zeros(3)
+    lib().ob_track_puncture(len(boxes), _boxes(boxes), pb, pp, dt, order, po.ctypes.data_as(DP), pn.ctypes.data_as(DP))
+    return pn

@@ -160,6 +160,16 @@ int ob_c2p_bracketed_pt(const ob_c2p_params *P, double D, double S2, double T, d
 void ob_c2p_bracketed(const ob_c2p_params *P, long n, const double *cons, const double *metric,
                       double *prim, int *status, int *iters);
 
+/* moving-puncture tracking (puncture_oracle.c): interpolate_xyz_scalar on a
+ * level of nb boxes (var[b] = the variable on box b; found = 0 and value 0
+ * when no box holds the interpolation cube), and one step of
+ * track_moving_puncture_integrate with beta[3*nb] / beta_p[3*nb] the shift
+ * components of the current / previous time level, box by box */
+double ob_interp_point(int nb, const ob_box *boxes, const double *const *var, double x, double y, double z, int order,
+                       int *found);
+void ob_track_puncture(int nb, const ob_box *boxes, const double *const *beta, const double *const *beta_p, double dt,
+                       int order, const double pold[3], double pnew[3]);
+
 #ifdef __cplusplus
 }
 #endif
