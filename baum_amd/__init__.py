@@ -117,7 +117,7 @@ def lib():
     _sig(L, "bamhip_restrict_prolong_varlist", i, [vp, vp, i, IP, IP, i, i])
     _sig(L, "bamhip_bo_interp_begin", i, [vp, i, i, i, IP, IP, IP, IP, IP, IP])
     _sig(L, "bamhip_bo_interp_end", i, [vp, i, i, i, IP, IP])
-    _sig(L, "bamhip_puncture_track", i, [vp, IP, IP, d, i, i, vp])
+    _sig(L, "bamhip_puncture_track", i, [vp, IP, IP, d, i, i, i, vp])
     _sig(L, "bamhip_timer_enable", i, [i])
     _sig(L, "bamhip_timer_reset", i, [])
     _sig(L, "bamhip_timer_get", i, [C.c_char_p, DP, C.POINTER(C.c_long)])
@@ -304,11 +304,11 @@ class Level:
                                            _iarr(upp) if upp is not None else None, timeinterpolation, iteration,
                                            1 if algcon else 0), "bamhip_z4c_rk4_step_bo")
 
-    def puncture_track(self, beta, beta_p, dt, pos, order=4):
+    def puncture_track(self, beta, beta_p, dt, pos, order=4, pin_z=False):
         """one tracking step of the punctures whose coordinates are in the
         device array pos (shape (np, 3)), updated in place"""
-        check(lib().bamhip_puncture_track(self.h, _iarr(beta), _iarr(beta_p), dt, order, pos.shape[0], pos.ptr),
-              "bamhip_puncture_track")
+        check(lib().bamhip_puncture_track(self.h, _iarr(beta), _iarr(beta_p), dt, order, 1 if pin_z else 0,
+                                          pos.shape[0], pos.ptr), "bamhip_puncture_track")
 
     def z4c_algcon(self, g, A, subtractA=1, normalizedetg=1):
         check(lib().bamhip_z4c_algcon(self.h, _iarr(g), _iarr(A), subtractA, normalizedetg), "bamhip_z4c_algcon")
@@ -367,7 +367,8 @@ class FmrHierarchyStruct(C.Structure):
     _fields_ = [("lmax", C.c_int), ("levels", C.POINTER(C.c_void_p)), ("params", C.POINTER(Z4cParams)),
                 ("dt0", C.c_double), ("iteration", C.POINTER(C.c_int)),
                 ("u", IP), ("up", IP), ("upp", IP), ("r", IP), ("v", IP),
-                ("order_RP", C.c_int), ("timeinterpolation", C.c_int), ("pgh", C.c_int)]
+                ("order_RP", C.c_int), ("timeinterpolation", C.c_int), ("pgh", C.c_int),
+                ("npunc", C.c_int), ("punc", C.c_void_p), ("punc_flags", C.c_int)]
 
 
 class Hierarchy:
@@ -375,7 +376,8 @@ class Hierarchy:
     levels[l] are Level objects, linked with set_rp_flags beforehand;
     params: one Z4cParams per level."""
 
-    def __init__(self, levels, params, dt0, u, up, upp, r, v, order_RP=4, timeinterpolation=3, pgh=0):
+    def __init__(self, levels, params, dt0, u, up, upp, r, v, order_RP=4, timeinterpolation=3, pgh=0,
+                 punctures=None, pin_z=False):
         L = lib()
         hp = C.POINTER(FmrHierarchyStruct)
         _sig(L, "bamhip_advance_levelandsublevels", C.c_int, [hp, C.c_int])
@@ -399,6 +401,10 @@ class Hierarchy:
         s.iteration = C.cast(self.iteration, C.POINTER(C.c_int))
         s.u, s.up, s.upp, s.r, s.v = [C.cast(a, IP) for a in self._lists]
         s.order_RP, s.timeinterpolation, s.pgh = order_RP, timeinterpolation, pgh
+        # punctures: device array (DeviceArray, shape (np, 3)) tracked on the finest level
+        self.punctures = punctures
+        if punctures is not None:
+            s.npunc, s.punc, s.punc_flags = punctures.shape[0], punctures.ptr, 1 if pin_z else 0
         self.s = s
 
     def set_check(self, mode):

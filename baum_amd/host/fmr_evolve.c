@@ -8,7 +8,9 @@
  * 116-132) after each step of a level with a parent: the parent's time-
  * interpolated u_pp is the coarse list unless both levels are at the same
  * time (parentaligned, inject.c:10-19).  Subcycling on every level (amr =
- * bo, lbo = 0).
+ * bo, lbo = 0).  After the finest level's step and its restriction, the
+ * puncture tracker of analyze_level (main.c:419-423; track_puncture,
+ * moving_punctures.c:111-196).
  *
  * The whole recursion issues device work only: the levels run with the
  * non-finite check deferred, so a coarse step can be captured as a graph.
@@ -47,6 +49,13 @@ int bamhip_advance_levelandsublevels(const bamhip_fmr_hierarchy *H, int l)
     const int aligned = H->iteration[l] == 2 * H->iteration[l - 1];
     if ((e = bamhip_restrict_prolong_varlist(H->levels[l - 1], H->levels[l], BAMHIP_Z4C_NVARS,
                                              aligned ? H->u : H->upp, H->u, 3, H->order_RP)))
+      return e;
+  }
+  /* analyze_level: the punctures move with the shift of the finest level */
+  if (l == H->lmax && H->npunc > 0) {
+    const double dt = H->dt0 / (double)(1L << l);
+    if ((e = bamhip_puncture_track(H->levels[l], H->u + 19, H->up + 19, dt, H->order_RP, H->punc_flags, H->npunc,
+                                   H->punc)))
       return e;
   }
   return BAMHIP_OK;

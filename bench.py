@@ -152,7 +152,11 @@ def run_fmr(args):
     # without it the 1/chi terms overflow at the punctures on levels >= 4
     par = B.z4c_params(z4_chi_div_floor=1e-5)
     dt0 = 0.25 * boxes[0][0].dx
-    H = B.Hierarchy(levels, [par] * len(levels), dt0, U, UP, UPP, R, V, order_RP=4, timeinterpolation=3)
+    # the punctures are tracked on the finest level after each of its steps (analyze_level)
+    punc0 = np.array([c for _, c in BINARY], dtype=np.float64)
+    punc = B.DeviceArray.from_host(punc0)
+    H = B.Hierarchy(levels, [par] * len(levels), dt0, U, UP, UPP, R, V, order_RP=4, timeinterpolation=3,
+                    punctures=punc)
     H.set_check(0)
     updates = sum(2 ** l * inner_points(bl) for l, bl in enumerate(boxes))
     # warm-up: the first coarse step eagerly (reduced-order time interpolation), then capture the next
@@ -181,13 +185,16 @@ def run_fmr(args):
     B.synchronize()
     B.timer_enable(False)
     split = {k: B.timer_get(k)[0] for k in ("z4_rhs", "z4_rhs_edge", "rk4_update", "vladd", "algcon",
-                                            "set_boundary", "prolong", "restrict")}
+                                            "set_boundary", "prolong", "restrict", "puncture")}
     desc = [[(b.m, b.n, b.o) for b in bl] for bl in boxes]
+    punc_end = punc.get()
+    punc.free()
     for lv in levels:
         lv.free()
     el = t1 - t0
     return dict(value=updates * args.steps / el, ms_per_step=1e3 * el / args.steps, updates_per_step=updates,
-                nonfinite=bool(bad), boxes=desc, eager_split_ms=split)
+                nonfinite=bool(bad), boxes=desc, eager_split_ms=split,
+                punctures_end=[[float("%.9g" % c) for c in p] for p in punc_end])
 
 
 # ------------------------------------------------------------------ box192
@@ -425,11 +432,11 @@ def main():
         out.update(value=fmr["value"], unit="grid-point-updates/s", ms_per_step=fmr["ms_per_step"])
         out["config"] = {"workload": "Z4c BO evolution of a 7-level FMR hierarchy: 128^3 level 0 + 2 boxes of "
                                      "48..96^3 per level 1..6 around a BL binary (m=0.5 at x=+-3), nbuffer 6, "
-                                     "order_RP 4, time interpolation 3, z4_chi_div_floor 1e-5, KO 0.1, "
+                                     "order_RP 4, time interpolation 3, z4_chi_div_floor 1e-5, KO 0.1, puncture tracking on level 6, "
                                      "one hipGraph per coarse step",
                          "updates_per_coarse_step": fmr["updates_per_step"], "boxes": fmr["boxes"],
                          "parallelism": "single GPU"}
-        out["fmr"] = {k: fmr[k] for k in ("nonfinite", "eager_split_ms")}
+        out["fmr"] = {k: fmr[k] for k in ("nonfinite", "eager_split_ms", "punctures_end")}
     elif box:
         out.update(value=box["value"], unit="grid-point-updates/s", ms_per_step=box["ms_per_step"])
         out["config"] = {"workload": "Z4c RK4, one 192^3 periodic box", "parallelism": "single GPU"}

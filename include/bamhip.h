@@ -341,7 +341,11 @@ int bamhip_bo_interp_end(bamhip_level *lev, int timeinterpolation, int iteration
  * state, the parent's time-interpolated state and the RK4 registers, the
  * same on every level.  params[l]: the RHS parameters of level l (the
  * dissipation factor may differ per level).  pgh: periodic ghosts of level 0
- * (0: radiative boundary). */
+ * (0: radiative boundary).  npunc > 0: punc is a device array of 3*npunc
+ * puncture coordinates, tracked on level lmax after each of its steps
+ * (track_puncture in analyze_level, main.c:415-423, with lmaxpunc = lmax)
+ * with the shift u[19..21] / up[19..21], interpolation order order_RP and
+ * punc_flags as bamhip_puncture_track's flags. */
 typedef struct {
   int lmax;
   bamhip_level *const *levels;
@@ -352,13 +356,16 @@ typedef struct {
   int order_RP;            /* prolongation / restriction order (4) */
   int timeinterpolation;   /* order_timeinterpolation: 2 or 3 */
   int pgh;
+  int npunc;               /* 0: no tracking */
+  double *punc;            /* device, 3*npunc */
+  int punc_flags;
 } bamhip_fmr_hierarchy;
 
 /* Replaces advance_levelandsublevels(g, l) (src/main/main/main.c:397-453)
  * for the Z4c list: one step of level l (time interpolation of u_pp around
  * an RK4 step, evolve.c:298-380, then algcon), two steps of every finer
- * level, then restrict_prolong_evolve with the parent (inject.c:116-132).
- * Device work only: the RHS non-finite check is deferred (levels must be in
+ * level, then restrict_prolong_evolve with the parent (inject.c:116-132),
+ * then on level lmax the puncture tracker.  Device work only: the RHS non-finite check is deferred (levels must be in
  * check mode 0, see bamhip_fmr_set_check), so the call can be captured. */
 int bamhip_advance_levelandsublevels(const bamhip_fmr_hierarchy *H, int l);
 int bamhip_fmr_set_check(const bamhip_fmr_hierarchy *H, int mode);
@@ -416,9 +423,13 @@ int bamhip_z4c_set_rk_fusion(int mode);
  * Lagrange of `order` points, 0 where no box holds the cube).  beta, beta_p:
  * 3 variable indices of the shift at the new and the previous time level;
  * pos: device array of 3*np coordinates, updated in place (np <= 64), so the
- * tracker can run inside a captured graph. */
-int bamhip_puncture_track(bamhip_level *lev, const int *beta, const int *beta_p, double dt, int order, int np,
-                          double *pos);
+ * tracker can run inside a captured graph.  As track_puncture
+ * (moving_punctures.c:111-196) does around it, a new position outside every
+ * box of the level is dropped for the old one, and flags
+ * BAMHIP_PUNCTURE_PIN_Z keeps z (moving_puncture_fixz = BH / BHNS). */
+#define BAMHIP_PUNCTURE_PIN_Z 1
+int bamhip_puncture_track(bamhip_level *lev, const int *beta, const int *beta_p, double dt, int order, int flags,
+                          int np, double *pos);
 
 /* --------------------------------------------------------------- graphs */
 /* Capture the device work issued on the library stream between begin and

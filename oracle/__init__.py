@@ -79,7 +79,7 @@ def lib():
                                       C.c_int, C.POINTER(C.c_int)]
         L.ob_track_puncture.restype = None
         L.ob_track_puncture.argtypes = [C.c_int, C.POINTER(Box), C.POINTER(DP), C.POINTER(DP), C.c_double, C.c_int,
-                                        DP, DP]
+                                        C.c_int, DP, DP]
         L.ob_z4c_ham_l2.restype = C.c_double
         L.ob_z4c_ham_l2.argtypes = [C.POINTER(Box), C.POINTER(DP), C.c_double, DP]
         L.ob_z4c_constraints_l2.restype = None
@@ -386,11 +386,14 @@ def interp_point(boxes, var, x, y, z, order=4):
     return v, bool(f.value)
 
 
-def track_puncture(boxes, beta, beta_p, dt, pold, order=4):
-    """one step of track_moving_puncture_integrate; beta, beta_p: 3 lists over boxes"""
+def track_puncture(boxes, beta, beta_p, dt, pold, order=4, pin_z=False):
+    """one step of track_moving_puncture_integrate inside track_puncture
+    (old position kept when the new one leaves the level; pin_z keeps z);
+    beta, beta_p: 3 lists over boxes"""
     pb, kb = _box_ptrs(beta)
     pp, kp = _box_ptrs(beta_p)
     po = np.ascontiguousarray(pold, dtype=np.float64)
     pn = np.zeros(3)
-    lib().ob_track_puncture(len(boxes), _boxes(boxes), pb, pp, dt, order, po.ctypes.data_as(DP), pn.ctypes.data_as(DP))
+    lib().ob_track_puncture(len(boxes), _boxes(boxes), pb, pp, dt, order, 1 if pin_z else 0, po.ctypes.data_as(DP),
+                            pn.ctypes.data_as(DP))
     return pn

@@ -164,11 +164,13 @@ void ob_c2p_bracketed(const ob_c2p_params *P, long n, const double *cons, const 
  * level of nb boxes (var[b] = the variable on box b; found = 0 and value 0
  * when no box holds the interpolation cube), and one step of
  * track_moving_puncture_integrate with beta[3*nb] / beta_p[3*nb] the shift
- * components of the current / previous time level, box by box */
+ * components of the current / previous time level, box by box; flags bit 0
+ * keeps z (moving_puncture_fixz BH); a step that leaves every box keeps the
+ * old position (track_puncture) */
 double ob_interp_point(int nb, const ob_box *boxes, const double *const *var, double x, double y, double z, int order,
                        int *found);
 void ob_track_puncture(int nb, const ob_box *boxes, const double *const *beta, const double *const *beta_p, double dt,
-                       int order, const double pold[3], double pnew[3]);
+                       int order, int flags, const double pold[3], double pnew[3]);
 
 #ifdef __cplusplus
 }

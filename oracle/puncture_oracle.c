@@ -13,7 +13,10 @@
  * track_moving_puncture_integrate (src/projects/Gauge/
  * moving_punctures_integrate.c:29-85): three ICN iterations
  * x0 = x1 - dt * (beta(x0) + beta_p(x1)) / 2, beta_p the shift of the previous
- * time level.
+ * time level.  track_puncture (moving_punctures.c:174-185) around it: with
+ * moving_puncture_fixz BH or BHNS the z coordinate stays; a new position that
+ * lies in no box's bounding box (xyzinsidelevel, amr/points.c:96-133, dless
+ * with dequaleps) is dropped and the old one kept.
  */
 #include <math.h>
 
@@ -77,8 +80,21 @@ double ob_interp_point(int nb, const ob_box *boxes, const double *const *var, do
   return 0.0;
 }
 
+static int inside_level(int nb, const ob_box *boxes, double x, double y, double z)
+{
+  for (int b = 0; b < nb; b++) {
+    const ob_box *B = &boxes[b];
+    const double x1 = B->x0 + (B->m - 1) * B->dx, y1 = B->y0 + (B->n - 1) * B->dy, z1 = B->z0 + (B->o - 1) * B->dz;
+    if (x < B->x0 - OB_DEQUALEPS || y < B->y0 - OB_DEQUALEPS || z < B->z0 - OB_DEQUALEPS || x1 < x - OB_DEQUALEPS ||
+        y1 < y - OB_DEQUALEPS || z1 < z - OB_DEQUALEPS)
+      continue;
+    return 1;
+  }
+  return 0;
+}
+
 void ob_track_puncture(int nb, const ob_box *boxes, const double *const *beta, const double *const *beta_p, double dt,
-                       int order, const double pold[3], double pnew[3])
+                       int order, int flags, const double pold[3], double pnew[3])
 {
   int f;
   const double x1 = pold[0], y1 = pold[1], z1 = pold[2];
@@ -93,6 +109,12 @@ void ob_track_puncture(int nb, const ob_box *boxes, const double *const *beta, c
     x0 = x1 - dt * 0.5 * (bx0 + bx1);
     y0 = y1 - dt * 0.5 * (by0 + by1);
     z0 = z1 - dt * 0.5 * (bz0 + bz1);
+  }
+  if (flags & 1) z0 = z1;
+  if (!inside_level(nb, boxes, x0, y0, z0)) {
+    x0 = x1;
+    y0 = y1;
+    z0 = z1;
   }
   pnew[0] = x0;
   pnew[1] = y0;

@@ -24,7 +24,7 @@ def _initial(boxes):
     return [np.concatenate([puncture_z4c(grid(b), BINARY).reshape(25, -1) for b in lev], axis=1) for lev in boxes]
 
 
-def _gpu_hierarchy(dt0):
+def _gpu_hierarchy(dt0, punctures=None):
     mk0 = lambda *a: B.make_box(*a, bflag=(B.PHYBOUND,) * 6)
     mkf = lambda *a: B.make_box(*a, bflag=(B.REFBOUND,) * 6)
     boxes, parents = small_hierarchy(mk0, mkf)
@@ -40,7 +40,8 @@ def _gpu_hierarchy(dt0):
     for l in range(1, len(levels)):
         B.set_rp_flags(levels[l - 1], levels[l], parents[l], NB, FP, FR)
     par = B.z4c_params()
-    H = B.Hierarchy(levels, [par] * len(levels), dt0, U, UP, UPP, R, V, order_RP=ORDER, timeinterpolation=TI)
+    H = B.Hierarchy(levels, [par] * len(levels), dt0, U, UP, UPP, R, V, order_RP=ORDER, timeinterpolation=TI,
+                    punctures=punctures)
     H.set_check(0)
     return H, levels
 
@@ -49,7 +50,8 @@ class OracleHierarchy:
     """the same recursion with the CPU oracle (main.c:397-453, evolve.c:298-380,
     inject.c:116-132)"""
 
-    def __init__(self, dt0):
+    def __init__(self, dt0, punctures=None):
+        self.punc = None if punctures is None else np.array(punctures, dtype=np.float64)
         self.boxes, self.parents = small_hierarchy(O.make_box, O.make_box)
         self.u = _initial(self.boxes)
         self.upp = [np.zeros_like(u) for u in self.u]
@@ -94,6 +96,17 @@ class OracleHierarchy:
             args = (self.boxes[l], self.boxes[l - 1], self.parents[l])
             assert O.restrict_level(*args, fr, 3, ORDER, self.u[l], cv) == 0
             assert O.prolong_level(*args, fp, 3, NB, ORDER, cv, self.u[l]) == 0
+        if l == len(self.boxes) - 1 and self.punc is not None:
+            self._track(l)
+
+    def _track(self, l):
+        """analyze_level on the finest level: track_puncture with the shift of
+        u and of u_p (the state at the start of the step)"""
+        offs = np.cumsum([0] + [b.m * b.n * b.o for b in self.boxes[l]])
+        beta = [[self.u[l][19 + a, offs[b]:offs[b + 1]] for b in range(len(self.boxes[l]))] for a in range(3)]
+        beta_p = [[self.work[l][b][19 + a].reshape(-1) for b in range(len(self.boxes[l]))] for a in range(3)]
+        for p in range(len(self.punc)):
+            self.punc[p] = O.track_puncture(self.boxes[l], beta, beta_p, self.dt0 / 2 ** l, self.punc[p], order=ORDER)
 
 
 def _rel_err(a, b):
@@ -120,10 +133,33 @@ def test_hierarchy_step_matches_oracle():
     assert np.abs(levels[2].get(U[18]) - u0[2][18]).max() > 1e-4
 
 
+PUNC0 = np.array([[-3.55, 0.07, 0.03], [3.04, -0.05, 0.02]])
+
+
+def test_hierarchy_puncture_tracking_matches_oracle():
+    """the tracker runs on the finest level after each of its steps (and the
+    restriction into its parent); the positions follow the oracle recursion
+    to round-off of the evolved shift"""
+    dt0 = 0.125
+    dp = B.DeviceArray.from_host(PUNC0)
+    H, levels = _gpu_hierarchy(dt0, punctures=dp)
+    ref = OracleHierarchy(dt0, punctures=PUNC0)
+    for step in range(2):
+        H.advance(0)
+        ref.advance(0)
+        got = dp.get()
+        np.testing.assert_allclose(got, ref.punc, rtol=0, atol=1e-12)
+    assert np.abs(got - PUNC0).max() > 1e-6          # the shift moved them
+    dp.free()
+    for lv in levels:
+        lv.free()
+
+
 def test_graph_replay_matches_eager():
     dt0 = 0.125
-    HA, LA = _gpu_hierarchy(dt0)
-    HB, LB = _gpu_hierarchy(dt0)
+    pa, pb = B.DeviceArray.from_host(PUNC0), B.DeviceArray.from_host(PUNC0)
+    HA, LA = _gpu_hierarchy(dt0, punctures=pa)
+    HB, LB = _gpu_hierarchy(dt0, punctures=pb)
     HA.advance(0)            # first step: time interpolation of reduced order
     HB.advance(0)
     g = HA.capture(0)        # records the second coarse step, executes nothing
@@ -138,9 +174,11 @@ def test_graph_replay_matches_eager():
             for la, lb in zip(LA, LB):
                 np.testing.assert_array_equal(la.get_state(U), lb.get_state(U))
                 np.testing.assert_array_equal(la.get_state(UPP), lb.get_state(UPP))
+            np.testing.assert_array_equal(pa.get(), pb.get())
     finally:
         B.Hierarchy.free_graph(g)
     assert HA.nonfinite() == 0
+    assert np.abs(pa.get() - PUNC0).max() > 1e-6
 
 
 @pytest.mark.parametrize("iteration", [0, 1])

@@ -57,3 +57,36 @@ def test_puncture_track_matches_oracle_bitwise(order):
     np.testing.assert_array_equal(got[3], pos0[3])    # outside the level: shift 0
     dpos.free()
     lev.free()
+
+
+@pytest.mark.parametrize("pin_z", [False, True])
+def test_puncture_track_pin_z_and_leaving_the_level(pin_z):
+    """large steps that carry some punctures out of the level (their old
+    positions stay) and pin_z (moving_puncture_fixz BH) against the oracle"""
+    h = 0.25
+    shapes = [(20, 18, 16, (-2.375, -2.125, -1.875)), (12, 14, 10, (2.625, -2.125, -1.875))]
+    oboxes = [O.make_box(m, n, o, *x0, h) for m, n, o, x0 in shapes]
+    gboxes = [B.make_box(m, n, o, *x0, h, bflag=(B.REFBOUND,) * 6) for m, n, o, x0 in shapes]
+    crd = [_coords(b) for b in oboxes]
+    beta = [[f.reshape(-1) for f in _shift(*crd[k], 1.0)] for k in range(2)]
+    beta_p = [[f.reshape(-1) for f in _shift(*crd[k], 0.9)] for k in range(2)]
+    lev = B.Level(gboxes, 25)
+    lev.enable(range(25))
+    for a in range(3):
+        lev.put(BETA[a], np.concatenate([beta[k][a] for k in range(2)]))
+        lev.put(BETA_P[a], np.concatenate([beta_p[k][a] for k in range(2)]))
+    rng = np.random.default_rng(11)
+    pos0 = np.column_stack([rng.uniform(-2.0, 4.8, 24), rng.uniform(-1.6, 1.6, 24), rng.uniform(-1.4, 1.4, 24)])
+    dpos = B.DeviceArray.from_host(pos0)
+    lev.puncture_track(BETA, BETA_P, 1.5, dpos, order=4, pin_z=pin_z)
+    got = dpos.get()
+    want = np.array([O.track_puncture(oboxes, [[beta[k][a] for k in range(2)] for a in range(3)],
+                                      [[beta_p[k][a] for k in range(2)] for a in range(3)], 1.5, p, order=4,
+                                      pin_z=pin_z) for p in pos0])
+    np.testing.assert_array_equal(got, want)
+    kept = np.all(got == pos0, axis=1)
+    assert 0 < kept.sum() < len(pos0)          # some left the level and stayed put, some moved
+    if pin_z:
+        np.testing.assert_array_equal(got[:, 2], pos0[:, 2])
+    dpos.free()
+    lev.free()

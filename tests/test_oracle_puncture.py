@@ -83,3 +83,24 @@ def test_icn_constant_and_linear_shift():
     for _ in range(2):
         x2 = x1 - dt * 0.5 * ((a + bb * x2) + b1)
     assert np.abs(x2 - got).max() > 1e-6
+
+
+def test_pin_z_and_step_out_of_the_level():
+    """track_puncture around the integrator (moving_punctures.c:174-185): with
+    moving_puncture_fixz BH the z coordinate stays; a new position in no box's
+    bounding box is dropped for the old one (constant shift: closed form)"""
+    boxes, h = _level()
+    dt = 0.1
+    p = np.array([0.37, -0.21, 0.13])
+    beta = [[np.full(b.m * b.n * b.o, c) for b in boxes] for c in (0.3, -0.2, 0.1)]
+    got = O.track_puncture(boxes, beta, beta, dt, p, pin_z=True)
+    np.testing.assert_allclose(got, [p[0] - dt * 0.3, p[1] + dt * 0.2, p[2]], rtol=0, atol=1e-15)
+    assert got[2] == p[2]
+    # the level spans x in [-2.375, 5.375]; where no box holds the cube the
+    # shift is 0, so with dt = 4 from x = -1.9 the ICN trial points are -3.1,
+    # then -1.9 - dt * 0.3 / 2 = -2.5 twice: outside, the old position stays
+    out = O.track_puncture(boxes, beta, beta, 4.0, np.array([-1.9, 0.1, 0.0]))
+    np.testing.assert_array_equal(out, [-1.9, 0.1, 0.0])
+    # from x = -1.2 with dt = 2 every trial point keeps its cube: x - dt beta
+    inside = O.track_puncture(boxes, beta, beta, 2.0, np.array([-1.2, 0.1, 0.0]))
+    np.testing.assert_allclose(inside, [-1.2 - 0.6, 0.1 + 0.4, -0.2], rtol=0, atol=1e-14)
