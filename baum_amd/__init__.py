@@ -310,6 +310,14 @@ class Level:
         check(lib().bamhip_puncture_track(self.h, _iarr(beta), _iarr(beta_p), dt, order, 1 if pin_z else 0,
                                           pos.shape[0], pos.ptr), "bamhip_puncture_track")
 
+    def grhd_fluxes(self, par, unew, metric, prim, c=0.0):
+        """matter_fluxes: unew (D, tau, S_i) -= flux divergence; metric: alpha,
+        beta^i, ADM g_ij (10 indices); prim: rho, epsl, v^i (5 indices)"""
+        L = lib()
+        _sig(L, "bamhip_grhd_fluxes", C.c_int, [C.c_void_p, C.POINTER(HrscParams), IP, C.c_double, IP, IP])
+        check(L.bamhip_grhd_fluxes(self.h, C.byref(par), _iarr(unew), c, _iarr(metric), _iarr(prim)),
+              "bamhip_grhd_fluxes")
+
     def z4c_algcon(self, g, A, subtractA=1, normalizedetg=1):
         check(lib().bamhip_z4c_algcon(self.h, _iarr(g), _iarr(A), subtractA, normalizedetg), "bamhip_z4c_algcon")
 
@@ -449,6 +457,17 @@ class C2PParams(C.Structure):
                 ("rhoatm", C.c_double), ("fatm", C.c_double), ("epslatm", C.c_double),
                 ("patm", C.c_double), ("vmax", C.c_double), ("errormax", C.c_double),
                 ("countmax", C.c_int)]
+
+
+class HrscParams(C.Structure):
+    _fields_ = [("eos", C.c_int), ("gamma", C.c_double), ("K", C.c_double), ("vmax", C.c_double),
+                ("wlormax", C.c_double), ("riemann", C.c_int), ("rec", C.c_int)]
+
+
+def hrsc_params(eos="ideal", gamma=2.0, K=100.0, vmax=0.999, wlormax=1e5, riemann="LLF", rec="WENOZ"):
+    """bamhip_hrsc_params: eos_*, grhd_vmax, grhd_Wlor_max, hrsc_flux, hrsc_rec"""
+    return HrscParams(0 if eos == "ideal" else 1, gamma, K, vmax, wlormax, {"LLF": 0, "HLL": 1}[riemann],
+                      {"WENOZ": 0, "WENO5": 1}[rec])
 
 
 def c2p_params(eos="ideal", gamma=2.0, K=100.0, rhoatm=1e-20, fatm=100.0, vmax=0.999, errormax=1e-10,

@@ -415,6 +415,30 @@ int bamhip_z4c_bc_rk4_stage(bamhip_level *lev, const bamhip_z4c_params *par, con
                             const int *uc, const int *up, const int *vout, double wc, double wv, int pgh);
 int bamhip_z4c_set_rk_fusion(int mode);
 
+/* -------------------------------------------------------- GRHD fluxes */
+/* hrsc_flux / hrsc_rec / EOS choices of grhd_flx1d_fvpr (grhd.c:193-209,
+ * hrsc.c:10-100, eos.c:37-72); the metric is reconstructed with LAG4
+ * (hrsc_rec_metric), the atmosphere mask is off (its default). */
+typedef struct {
+  int eos;          /* 0 ideal gas, 1 polytrope (EOS.COLD: no tau flux) */
+  double gamma, K;  /* eos_Gamma, eos_K */
+  double vmax;      /* grhd_vmax */
+  double wlormax;   /* grhd_Wlor_max */
+  int riemann;      /* hrsc_flux: 0 LLF, 1 HLL */
+  int rec;          /* hrsc_rec: 0 WENOZ, 1 WENO5 */
+} bamhip_hrsc_params;
+
+/* Replaces matter_fluxes(unew, upre, c, ucur, primVars, otherVars)
+ * (src/projects/matter/matter_fluxes.c:20-86) with MATTER.hrsc_flx1d =
+ * grhd_flx1d_fvpr (grhd_flx1d.c:25-250): at every point of every box,
+ * unew -= (F_{i+1/2} - F_{i-1/2}) / h for x, then y, then z (times c when
+ * c != 0), lines extended by flat extrapolation, non-finite interface
+ * fluxes zeroed.  unew: 5 indices (D, tau, S_x, S_y, S_z); metric: 10
+ * (alpha, beta^x..z, ADM gxx, gxy, gxz, gyy, gyz, gzz); prim: 5 (rho,
+ * epsl, v^x..z; grhd.c:46-53). */
+int bamhip_grhd_fluxes(bamhip_level *lev, const bamhip_hrsc_params *par, const int *unew, double c,
+                       const int *metric, const int *prim);
+
 /* ---------------------------------------------------------- punctures */
 /* Replaces track_moving_puncture_integrate (src/projects/Gauge/
  * moving_punctures_integrate.c:29-85) on the finest level of a puncture: three

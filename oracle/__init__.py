@@ -55,6 +55,16 @@ class C2PParams(C.Structure):
                 ("countmax", C.c_int)]
 
 
+class HrscParams(C.Structure):
+    _fields_ = [("eos", C.c_int), ("gamma", C.c_double), ("K", C.c_double),
+                ("vmax", C.c_double), ("wlormax", C.c_double), ("riemann", C.c_int), ("rec", C.c_int)]
+
+
+def hrsc_params(eos=0, gamma=2.0, K=100.0, vmax=0.999, wlormax=1e5, riemann="LLF", rec="WENOZ"):
+    """grhd_vmax / grhd_Wlor_max / hrsc_flux / hrsc_rec of the reference's BNS parameter file"""
+    return HrscParams(eos, gamma, K, vmax, wlormax, {"LLF": 0, "HLL": 1}[riemann], {"WENOZ": 0, "WENO5": 1}[rec])
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -74,6 +84,13 @@ def lib():
         L.ob_z4c_rk4_step.restype = C.c_int
         L.ob_z4c_rk4_step.argtypes = [C.POINTER(Box), C.POINTER(Z4cParams), C.c_double, C.c_int,
                                       C.c_int, C.POINTER(DP), C.POINTER(DP), C.c_int]
+        L.ob_hrsc_rec.restype = C.c_double
+        L.ob_hrsc_rec.argtypes = [C.c_int, C.c_int, DP, C.c_int]
+        L.ob_grhd_interface_flux.restype = None
+        L.ob_grhd_interface_flux.argtypes = [C.POINTER(HrscParams), DP, DP, DP, DP]
+        L.ob_grhd_fluxes.restype = None
+        L.ob_grhd_fluxes.argtypes = [C.POINTER(Box), C.POINTER(HrscParams), C.c_double, C.POINTER(DP),
+                                     C.POINTER(DP), C.POINTER(DP)]
         L.ob_interp_point.restype = C.c_double
         L.ob_interp_point.argtypes = [C.c_int, C.POINTER(Box), C.POINTER(DP), C.c_double, C.c_double, C.c_double,
                                       C.c_int, C.POINTER(C.c_int)]
@@ -397,3 +414,30 @@ def track_puncture(boxes, beta, beta_p, dt, pold, order=4, pin_z=False):
     lib().ob_track_puncture(len(boxes), _boxes(boxes), pb, pp, dt, order, 1 if pin_z else 0, po.ctypes.data_as(DP),
                             pn.ctypes.data_as(DP))
     return pn
+
+
+# ---------------------------------------------------------------- GRHD fluxes
+def hrsc_rec(kind, side, u, i):
+    """kind "WENOZ" / "WENO5" / "LAG4"; side 0: value at i+1/2 from the left
+    (rec1d_p_*), 1: value at i-1/2 from the right (rec1d_m_*)"""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    return lib().ob_hrsc_rec({"WENOZ": 0, "WENO5": 1, "LAG4": 2}[kind], side, u.ctypes.data_as(DP), i)
+
+
+def grhd_interface_flux(par, g, wl, wr):
+    """grhd_flx1d_fvpr at one interface: g = (alpha, beta^x, gxx..gzz), w = (rho, epsl, vx, vy, vz)"""
+    g, wl, wr = (np.ascontiguousarray(a, dtype=np.float64) for a in (g, wl, wr))
+    F = np.zeros(5)
+    lib().ob_grhd_interface_flux(C.byref(par), g.ctypes.data_as(DP), wl.ctypes.data_as(DP),
+                                 wr.ctypes.data_as(DP), F.ctypes.data_as(DP))
+    return F
+
+
+def grhd_fluxes(box, par, metric, prim, rhs, c=0.0):
+    """matter_fluxes on one box: metric (10, o, n, m), prim (5, o, n, m);
+    rhs (5, o, n, m) is updated in place"""
+    metric = np.ascontiguousarray(metric, dtype=np.float64)
+    prim = np.ascontiguousarray(prim, dtype=np.float64)
+    assert rhs.dtype == np.float64 and rhs.flags.c_contiguous
+    lib().ob_grhd_fluxes(C.byref(box), C.byref(par), c, _ptrs(metric), _ptrs(prim), _ptrs(rhs))
+    return rhs

@@ -160,6 +160,25 @@ int ob_c2p_bracketed_pt(const ob_c2p_params *P, double D, double S2, double T, d
 void ob_c2p_bracketed(const ob_c2p_params *P, long n, const double *cons, const double *metric,
                       double *prim, int *status, int *iters);
 
+/* GRHD flux divergence (grhd_flux_oracle.c): matter_fluxes with
+ * grhd_flx1d_fvpr.  eos 0 ideal gas, 1 polytrope; riemann 0 LLF, 1 HLL;
+ * rec 0 WENOZ, 1 WENO5 (primitives; the metric is always LAG4). */
+typedef struct {
+  int eos;
+  double gamma, K;
+  double vmax, wlormax; /* grhd_vmax, grhd_Wlor_max */
+  int riemann, rec;
+} ob_hrsc_params;
+/* kind 0 WENOZ, 1 WENO5, 2 LAG4; side 0 rec1d_p_*(u, i), 1 rec1d_m_*(u, i) */
+double ob_hrsc_rec(int kind, int side, const double *u, int i);
+/* g: alpha, beta^x, gxx..gzz at the interface; wl/wr: rho, epsl, v^x,y,z */
+void ob_grhd_interface_flux(const ob_hrsc_params *P, const double g[8], const double wl[5], const double wr[5],
+                            double F[5]);
+/* metric: alpha, beta^x,y,z, gxx, gxy, gxz, gyy, gyz, gzz (10 arrays);
+ * prim: rho, epsl, v^x,y,z; rhs: D, tau, S_x,y,z, updated in place */
+void ob_grhd_fluxes(const ob_box *B, const ob_hrsc_params *P, double c, const double *const *metric,
+                    const double *const *prim, double *const *rhs);
+
 /* moving-puncture tracking (puncture_oracle.c): interpolate_xyz_scalar on a
  * level of nb boxes (var[b] = the variable on box b; found = 0 and value 0
  * when no box holds the interpolation cube), and one step of
