@@ -318,6 +318,15 @@ class Level:
         check(L.bamhip_grhd_fluxes(self.h, C.byref(par), _iarr(unew), c, _iarr(metric), _iarr(prim)),
               "bamhip_grhd_fluxes")
 
+    def grhd_sources(self, par, unew, metric, Kij, prim, madm, c=0.0, rhoatm=0.0, order_centered=4):
+        """grhd_sources: unew (D, tau, S_i) += source terms at the inner
+        points; madm (rho, S_i, S_ij, S) written"""
+        L = lib()
+        _sig(L, "bamhip_grhd_sources", C.c_int, [C.c_void_p, C.POINTER(HrscParams), C.c_double, C.c_int, IP,
+                                                  C.c_double, IP, IP, IP, IP])
+        check(L.bamhip_grhd_sources(self.h, C.byref(par), rhoatm, order_centered, _iarr(unew), c, _iarr(metric),
+                                    _iarr(Kij), _iarr(prim), _iarr(madm)), "bamhip_grhd_sources")
+
     def z4c_algcon(self, g, A, subtractA=1, normalizedetg=1):
         check(lib().bamhip_z4c_algcon(self.h, _iarr(g), _iarr(A), subtractA, normalizedetg), "bamhip_z4c_algcon")
 

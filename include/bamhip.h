@@ -439,6 +439,19 @@ typedef struct {
 int bamhip_grhd_fluxes(bamhip_level *lev, const bamhip_hrsc_params *par, const int *unew, double c,
                        const int *metric, const int *prim);
 
+/* Replaces MATTER.sources = grhd_sources(unew, upre, c, ucur, primVars,
+ * otherVars, matterADMVars) (src/projects/grhd/grhd_sources.m:8-190,
+ * generated grhd_sources.c; called at matter.c:438-442) with
+ * grhd_source_computation = prim and grhd_cooling = 0 (bam_grhd.c:51-52):
+ * at the inner points, unew (D, tau, S_i) += the geometric source terms
+ * (times c when c != 0) and madm (11 indices: rho, S_i, S_ij, S) is written.
+ * metric: 10 (alpha, beta^i, ADM g_ij), Kij: 6 (ADM K_ij), prim: 6 (rho,
+ * epsl, v^i, p).  rhoatm: GRHD.ATM_RHOATM (no momentum source below it);
+ * par->eos 1 (cold EOS): no tau source; order_centered 2 or 4. */
+int bamhip_grhd_sources(bamhip_level *lev, const bamhip_hrsc_params *par, double rhoatm, int order_centered,
+                        const int *unew, double c, const int *metric, const int *Kij, const int *prim,
+                        const int *madm);
+
 /* ---------------------------------------------------------- punctures */
 /* Replaces track_moving_puncture_integrate (src/projects/Gauge/
  * moving_punctures_integrate.c:29-85) on the finest level of a puncture: three

@@ -91,6 +91,9 @@ def lib():
         L.ob_grhd_fluxes.restype = None
         L.ob_grhd_fluxes.argtypes = [C.POINTER(Box), C.POINTER(HrscParams), C.c_double, C.POINTER(DP),
                                      C.POINTER(DP), C.POINTER(DP)]
+        L.ob_grhd_sources.restype = None
+        L.ob_grhd_sources.argtypes = [C.POINTER(Box), C.POINTER(HrscParams), C.c_double, C.c_int, C.c_double,
+                                      C.POINTER(DP), C.POINTER(DP), C.POINTER(DP), C.POINTER(DP), C.POINTER(DP)]
         L.ob_interp_point.restype = C.c_double
         L.ob_interp_point.argtypes = [C.c_int, C.POINTER(Box), C.POINTER(DP), C.c_double, C.c_double, C.c_double,
                                       C.c_int, C.POINTER(C.c_int)]
@@ -441,3 +444,15 @@ def grhd_fluxes(box, par, metric, prim, rhs, c=0.0):
     assert rhs.dtype == np.float64 and rhs.flags.c_contiguous
     lib().ob_grhd_fluxes(C.byref(box), C.byref(par), c, _ptrs(metric), _ptrs(prim), _ptrs(rhs))
     return rhs
+
+
+def grhd_sources(box, par, metric, Kij, prim, rhs, madm, c=0.0, rhoatm=0.0, order_centered=4):
+    """grhd_sources on one box: metric (10, o, n, m), Kij (6, ...), prim (6,
+    ...: rho, epsl, v^i, p); rhs (5, ...) updated and madm (11, ...) written
+    in place at the inner points"""
+    metric, Kij, prim = (np.ascontiguousarray(a, dtype=np.float64) for a in (metric, Kij, prim))
+    for a in (rhs, madm):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    lib().ob_grhd_sources(C.byref(box), C.byref(par), rhoatm, order_centered, c, _ptrs(metric), _ptrs(Kij),
+                          _ptrs(prim), _ptrs(rhs), _ptrs(madm))
+    return rhs, madm

@@ -179,6 +179,14 @@ void ob_grhd_interface_flux(const ob_hrsc_params *P, const double g[8], const do
 void ob_grhd_fluxes(const ob_box *B, const ob_hrsc_params *P, double c, const double *const *metric,
                     const double *const *prim, double *const *rhs);
 
+/* GRHD source terms and matter ADM variables (grhd_source_oracle.c),
+ * grhd_sources at the inner points.  metric: alpha, beta^x,y,z, g_ij (10);
+ * Kij: K_ij (6); prim: rho, epsl, v^x,y,z, p (6); rhs: D, tau, S_i (5,
+ * updated); madm: rho, S_i, S_ij, S (11, written).  P->eos 1: cold EOS. */
+void ob_grhd_sources(const ob_box *B, const ob_hrsc_params *P, double rhoatm, int order_centered, double c,
+                     const double *const *metric, const double *const *Kij, const double *const *prim,
+                     double *const *rhs, double *const *madm);
+
 /* moving-puncture tracking (puncture_oracle.c): interpolate_xyz_scalar on a
  * level of nb boxes (var[b] = the variable on box b; found = 0 and value 0
  * when no box holds the interpolation cube), and one step of
