@@ -327,6 +327,13 @@ class Level:
         check(L.bamhip_grhd_sources(self.h, C.byref(par), rhoatm, order_centered, _iarr(unew), c, _iarr(metric),
                                     _iarr(Kij), _iarr(prim), _iarr(madm)), "bamhip_grhd_sources")
 
+    def z4c_matter_source(self, par, unew, ucur, madm, c=0.0):
+        """z4_rhs_add_source: the matter terms of the Z4c right-hand sides"""
+        L = lib()
+        _sig(L, "bamhip_z4c_matter_source", C.c_int, [C.c_void_p, C.POINTER(Z4cParams), IP, C.c_double, IP, IP])
+        check(L.bamhip_z4c_matter_source(self.h, C.byref(par), _iarr(unew), c, _iarr(ucur), _iarr(madm)),
+              "bamhip_z4c_matter_source")
+
     def z4c_algcon(self, g, A, subtractA=1, normalizedetg=1):
         check(lib().bamhip_z4c_algcon(self.h, _iarr(g), _iarr(A), subtractA, normalizedetg), "bamhip_z4c_algcon")
 

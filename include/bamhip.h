@@ -452,6 +452,16 @@ int bamhip_grhd_sources(bamhip_level *lev, const bamhip_hrsc_params *par, double
                         const int *unew, double c, const int *metric, const int *Kij, const int *prim,
                         const int *madm);
 
+/* Replaces z4_rhs_add_source(unew, upre, c, ucur) (src/projects/z4/
+ * z4_rhs_add_source.m:14-77; "geometry_source" at z4.c:247, called after
+ * the matter RHS, evolve.c:72-76): at the inner points the matter terms of
+ * the A_ij, Khat, G^i, Theta and B^i right-hand sides from the matter ADM
+ * variables (madm: 11 indices rho, S^i, S_ij, S, as written by
+ * bamhip_grhd_sources), times c when c != 0.  unew/ucur: the 25 Z4c
+ * indices. */
+int bamhip_z4c_matter_source(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, double c,
+                             const int *ucur, const int *madm);
+
 /* ---------------------------------------------------------- punctures */
 /* Replaces track_moving_puncture_integrate (src/projects/Gauge/
  * moving_punctures_integrate.c:29-85) on the finest level of a puncture: three

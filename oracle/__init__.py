@@ -94,6 +94,9 @@ def lib():
         L.ob_grhd_sources.restype = None
         L.ob_grhd_sources.argtypes = [C.POINTER(Box), C.POINTER(HrscParams), C.c_double, C.c_int, C.c_double,
                                       C.POINTER(DP), C.POINTER(DP), C.POINTER(DP), C.POINTER(DP), C.POINTER(DP)]
+        L.ob_z4c_matter_source.restype = None
+        L.ob_z4c_matter_source.argtypes = [C.POINTER(Box), C.POINTER(Z4cParams), C.c_double, C.POINTER(DP),
+                                           C.POINTER(DP), C.POINTER(DP)]
         L.ob_interp_point.restype = C.c_double
         L.ob_interp_point.argtypes = [C.c_int, C.POINTER(Box), C.POINTER(DP), C.c_double, C.c_double, C.c_double,
                                       C.c_int, C.POINTER(C.c_int)]
@@ -456,3 +459,12 @@ def grhd_sources(box, par, metric, Kij, prim, rhs, madm, c=0.0, rhoatm=0.0, orde
     lib().ob_grhd_sources(C.byref(box), C.byref(par), rhoatm, order_centered, c, _ptrs(metric), _ptrs(Kij),
                           _ptrs(prim), _ptrs(rhs), _ptrs(madm))
     return rhs, madm
+
+
+def z4c_matter_source(box, par, u, madm, r, c=0.0):
+    """z4_rhs_add_source on one box: u (25, o, n, m) Z4c fields, madm (11, ...)
+    matter ADM variables; r (25, ...) updated in place at the inner points"""
+    u, madm = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, madm))
+    assert r.dtype == np.float64 and r.flags.c_contiguous
+    lib().ob_z4c_matter_source(C.byref(box), C.byref(par), c, _ptrs(u), _ptrs(madm), _ptrs(r))
+    return r

@@ -187,6 +187,13 @@ void ob_grhd_sources(const ob_box *B, const ob_hrsc_params *P, double rhoatm, in
                      const double *const *metric, const double *const *Kij, const double *const *prim,
                      double *const *rhs, double *const *madm);
 
+/* matter terms of the Z4c right-hand side (z4c_matter_oracle.c),
+ * z4_rhs_add_source at the inner points: u the 25 Z4c fields, madm the 11
+ * matter ADM variables (rho, S^i, S_ij, S), r the 25 right-hand sides
+ * (A_ij, Khat, G^i, Theta, B^i updated) */
+void ob_z4c_matter_source(const ob_box *B, const ob_z4c_params *P, double c, const double *const *u,
+                          const double *const *madm, double *const *r);
+
 /* moving-puncture tracking (puncture_oracle.c): interpolate_xyz_scalar on a
  * level of nb boxes (var[b] = the variable on box b; found = 0 and value 0
  * when no box holds the interpolation cube), and one step of
