@@ -334,6 +334,25 @@ class Level:
         check(L.bamhip_z4c_matter_source(self.h, C.byref(par), _iarr(unew), c, _iarr(ucur), _iarr(madm)),
               "bamhip_z4c_matter_source")
 
+    def z4c_to_adm(self, u, adm_g, adm_K, chipsipower=-4.0):
+        """z4_to_adm_vl: ADM g_ij, K_ij from the Z4c fields"""
+        L = lib()
+        _sig(L, "bamhip_z4c_to_adm", C.c_int, [C.c_void_p, IP, C.c_double, IP, IP])
+        check(L.bamhip_z4c_to_adm(self.h, _iarr(u), chipsipower, _iarr(adm_g), _iarr(adm_K)), "bamhip_z4c_to_adm")
+
+    def z4c_grhd_rhs(self, z4, c2p, hrsc, unew, ucur, adm_g, adm_K, prim, madm, status, iters, upre=None, c=0.0,
+                     method=0, mlp=None, polish_max=8, fallbacks=None, pgh=0):
+        """evolve_rhs of the coupled Z4c + GRHD system; lists of 30 indices
+        (25 Z4c fields, then D, tau, S_i); status / iters: DeviceArray int32"""
+        L = lib()
+        _sig(L, "bamhip_z4c_grhd_rhs", C.c_int, [C.c_void_p, C.POINTER(Z4cGrhdStruct), IP, IP, C.c_double, IP])
+        keep = [_iarr(x) for x in (adm_g, adm_K, prim, madm)]
+        S = Z4cGrhdStruct(C.pointer(z4), C.pointer(c2p), C.pointer(hrsc), method, mlp.h if mlp is not None else None,
+                          polish_max, fallbacks.ptr if fallbacks is not None else None, pgh,
+                          *[C.cast(k, IP) for k in keep], status.ptr, iters.ptr)
+        check(L.bamhip_z4c_grhd_rhs(self.h, C.byref(S), _iarr(unew), _iarr(upre) if upre is not None else None, c,
+                                    _iarr(ucur)), "bamhip_z4c_grhd_rhs")
+
     def z4c_algcon(self, g, A, subtractA=1, normalizedetg=1):
         check(lib().bamhip_z4c_algcon(self.h, _iarr(g), _iarr(A), subtractA, normalizedetg), "bamhip_z4c_algcon")
 
@@ -484,6 +503,16 @@ def hrsc_params(eos="ideal", gamma=2.0, K=100.0, vmax=0.999, wlormax=1e5, rieman
     """bamhip_hrsc_params: eos_*, grhd_vmax, grhd_Wlor_max, hrsc_flux, hrsc_rec"""
     return HrscParams(0 if eos == "ideal" else 1, gamma, K, vmax, wlormax, {"LLF": 0, "HLL": 1}[riemann],
                       {"WENOZ": 0, "WENO5": 1}[rec])
+
+
+C2P_LEARNED_METHOD = 3
+
+
+class Z4cGrhdStruct(C.Structure):
+    _fields_ = [("z4", C.POINTER(Z4cParams)), ("c2p", C.POINTER(C2PParams)), ("hrsc", C.POINTER(HrscParams)),
+                ("c2p_method", C.c_int), ("mlp", C.c_void_p), ("polish_max", C.c_int), ("fallbacks", C.c_void_p),
+                ("pgh", C.c_int), ("adm_g", IP), ("adm_K", IP), ("prim", IP), ("madm", IP),
+                ("status", C.c_void_p), ("iters", C.c_void_p)]
 
 
 def c2p_params(eos="ideal", gamma=2.0, K=100.0, rhoatm=1e-20, fatm=100.0, vmax=0.999, errormax=1e-10,

@@ -88,10 +88,53 @@ __global__ void __launch_bounds__(kMBlock) z4c_matter_kernel(MatterArgs A) {
   for (int a = 0; a < 3; a++) A.rB[a][q] += cc * rB[a];
 }
 
+// z4_to_adm_vl (z4.c:93-124) at every point: psi = chi^(1/chipsipower),
+// g_ij = psi^4 g~_ij, K_ij = psi^4 A~_ij + Khat g_ij / 3 (the reference takes
+// the stored bssn_K, which is Khat)
+struct AdmArgs {
+  const double *g[6], *A[6], *chi, *Kh;
+  double *gadm[6], *kadm[6];
+  double oopower;
+  long n;
+};
+
+__global__ void __launch_bounds__(kMBlock) z4c_to_adm_kernel(AdmArgs A) {
+  const long q = (long)blockIdx.x * kMBlock + threadIdx.x;
+  if (q >= A.n) return;
+  const double psi = pow(A.chi[q], A.oopower);
+  const double psi4 = pow(psi, 4.);
+  const double kh = A.Kh[q];
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    const double gg = psi4 * A.g[c][q];
+    A.gadm[c][q] = gg;
+    A.kadm[c][q] = psi4 * A.A[c][q] + 1. / 3. * kh * gg;
+  }
+}
+
 }  // namespace
 }  // namespace bamhip
 
 using namespace bamhip;
+
+extern "C" int bamhip_z4c_to_adm(bamhip_level *lev, const int *ucur, double chipsipower, const int *adm_g,
+                                 const int *adm_K) {
+  if (!lev) return set_error(BAMHIP_EINVAL, "bamhip_z4c_to_adm: NULL level");
+  double *u[25], *g[6], *k[6];
+  int e;
+  if ((e = gather_ptrs(lev, ucur, BAMHIP_Z4C_NVARS, u, "bamhip_z4c_to_adm(ucur)"))) return e;
+  if ((e = gather_ptrs(lev, adm_g, 6, g, "bamhip_z4c_to_adm(adm_g)"))) return e;
+  if ((e = gather_ptrs(lev, adm_K, 6, k, "bamhip_z4c_to_adm(adm_K)"))) return e;
+  AdmArgs A{};
+  for (int c = 0; c < 6; c++) A.g[c] = u[c], A.A[c] = u[7 + c], A.gadm[c] = g[c], A.kadm[c] = k[c];
+  A.chi = u[6];
+  A.Kh = u[13];
+  A.oopower = 1. / chipsipower;
+  A.n = lev->npoints;
+  if (A.n > 0) z4c_to_adm_kernel<<<(unsigned)((A.n + kMBlock - 1) / kMBlock), kMBlock, 0, stream()>>>(A);
+  BH_HIP(hipGetLastError());
+  return BAMHIP_OK;
+}
 
 extern "C" int bamhip_z4c_matter_source(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, double c,
                                         const int *ucur, const int *madm) {

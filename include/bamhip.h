@@ -462,6 +462,37 @@ int bamhip_grhd_sources(bamhip_level *lev, const bamhip_hrsc_params *par, double
 int bamhip_z4c_matter_source(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, double c,
                              const int *ucur, const int *madm);
 
+/* Replaces z4_to_adm_vl(unew, upre, c, ucur) (src/projects/z4/z4.c:93-124),
+ * the "geometry_adm" step of evolve_rhs: at every point psi =
+ * chi^(1/chipsipower), adm_g = psi^4 g~_ij, adm_K = psi^4 A~_ij + Khat g_ij / 3.
+ * ucur: the 25 Z4c indices; adm_g, adm_K: 6 indices each (written). */
+int bamhip_z4c_to_adm(bamhip_level *lev, const int *ucur, double chipsipower, const int *adm_g, const int *adm_K);
+
+/* The coupled Z4c + GRHD right-hand side: evolve_rhs (src/utility/evolve/
+ * evolve.c:37-111) for physics z4 + matter + grhd as one call:
+ * bamhip_z4c_rhs, bamhip_z4c_to_adm, the c2p of c2p_method (BAMHIP_C2P_*
+ * on the hybrid/plain/bracketed solvers, or BAMHIP_C2P_LEARNED with mlp),
+ * matter_rhs_prepare, bamhip_grhd_sources, bamhip_grhd_fluxes,
+ * bamhip_z4c_matter_source and the Z4c boundary.  The lists unew / upre /
+ * ucur hold 30 indices: the 25 Z4c fields, then D, tau, S_x, S_y, S_z. */
+#define BAMHIP_C2P_LEARNED 3
+typedef struct {
+  const bamhip_z4c_params *z4;
+  const bamhip_c2p_params *c2p;    /* EOS, atmosphere (its rhoatm is GRHD.ATM_RHOATM) */
+  const bamhip_hrsc_params *hrsc;
+  int c2p_method;
+  const bamhip_mlp *mlp;           /* BAMHIP_C2P_LEARNED only */
+  int polish_max;
+  int *fallbacks;                  /* device int or NULL */
+  int pgh;                         /* periodic ghost width (0: radiative boundary) */
+  const int *adm_g, *adm_K;        /* 6 + 6 indices (adm_gxx.., adm_Kxx..), written */
+  const int *prim;                 /* 8: primVars (rho, epsl, v^i, p, v2, detg) */
+  const int *madm;                 /* 11: matter ADM variables, written */
+  int *status, *iters;             /* device arrays of npoints ints for the c2p */
+} bamhip_z4c_grhd;
+int bamhip_z4c_grhd_rhs(bamhip_level *lev, const bamhip_z4c_grhd *S, const int *unew, const int *upre, double c,
+                        const int *ucur);
+
 /* ---------------------------------------------------------- punctures */
 /* Replaces track_moving_puncture_integrate (src/projects/Gauge/
  * moving_punctures_integrate.c:29-85) on the finest level of a puncture: three

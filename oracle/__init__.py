@@ -468,3 +468,12 @@ def z4c_matter_source(box, par, u, madm, r, c=0.0):
     assert r.dtype == np.float64 and r.flags.c_contiguous
     lib().ob_z4c_matter_source(C.byref(box), C.byref(par), c, _ptrs(u), _ptrs(madm), _ptrs(r))
     return r
+
+
+def z4c_to_adm(box, u, chipsipower=-4.0):
+    """z4_to_adm_vl on one box: returns (g_ij (6, ...), K_ij (6, ...))"""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    g = np.zeros((6,) + u.shape[1:])
+    k = np.zeros((6,) + u.shape[1:])
+    lib().ob_z4c_to_adm(C.byref(box), chipsipower, _ptrs(u), _ptrs(g), _ptrs(k))
+    return g, k
