@@ -61,7 +61,7 @@ def parse():
     ap.add_argument("--rk-fusion", type=int, default=2,
                     help="RK4 stage update inside the RHS kernels: 0 never, 1 always, 2 on levels >= 2^22 points")
     ap.add_argument("--xcd-map", type=int, default=1, help="1: XCD-aware tile numbering in the RHS kernels")
-    ap.add_argument("--only", default="fmr,box192,c2p")
+    ap.add_argument("--only", default="fmr,box192,c2p,c5")
     return ap.parse_args()
 
 
@@ -369,7 +369,138 @@ def run_c2p(args):
     out["learned_vs_bracketed_speedup"] = out["learned"]["pts_per_s"] / out["bracketed"]["pts_per_s"]
     for a in (dc0, dg, dc, dpr, ds, di, fb):
         a.free()
-    net.free()
+    return out, net
+
+
+# ------------------------------------------------------------------ C5: coupled Z4c + GRHD
+def tov_isotropic(rhoc, K=100.0, G=2.0, dr=2e-3):
+    """TOV star of the polytrope p = K rho^G: RK4 in the Schwarzschild radius
+    for m, p, nu and the isotropic radius, rescaled to the exterior
+    Schwarzschild solution in isotropic coordinates at the surface"""
+    def rhs(r, y):
+        m, P = y[0], y[1]
+        rho = (max(P, 0.0) / K) ** (1.0 / G)
+        e = rho + P / (G - 1.0)
+        den = r * (r - 2 * m)
+        return np.array([4 * np.pi * r * r * e, -(e + P) * (m + 4 * np.pi * r ** 3 * P) / den,
+                         (m + 4 * np.pi * r ** 3 * P) / den, 1.0 / (r * np.sqrt(1 - 2 * m / r))])
+    Pc = K * rhoc ** G
+    r = dr
+    y = np.array([4 / 3 * np.pi * r ** 3 * (rhoc + Pc / (G - 1)), Pc, 0.0, np.log(r)])
+    rs, ys = [r], [y.copy()]
+    while True:
+        k1 = rhs(r, y)
+        k2 = rhs(r + dr / 2, y + dr / 2 * k1)
+        k3 = rhs(r + dr / 2, y + dr / 2 * k2)
+        k4 = rhs(r + dr, y + dr * k3)
+        yn = y + dr / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        if yn[1] <= 0:
+            break
+        r, y = r + dr, yn
+        rs.append(r)
+        ys.append(y.copy())
+    rs, ys = np.array(rs), np.array(ys)
+    R, M = rs[-1], ys[-1, 0]
+    rbs = 0.5 * (R - M + np.sqrt(R * (R - 2 * M)))
+    rb = np.exp(ys[:, 3] - ys[-1, 3]) * rbs
+    return dict(M=M, R=R, rbs=rbs, rb=np.concatenate([[0.0], rb]),
+                alpha=np.concatenate([[0.0], np.exp(ys[:, 2] - ys[-1, 2]) * np.sqrt(1 - 2 * M / R)]),
+                psi4=np.concatenate([[0.0], (rs / rb) ** 2]),
+                rho=np.concatenate([[rhoc], (np.maximum(ys[:, 1], 0) / K) ** (1 / G)]))
+
+
+def run_c5(args, net):
+    """one evaluation of the coupled Z4c + GRHD right-hand side on a 128^3
+    box holding a TOV star (Gamma 2, K 100, rho_c 1.28e-3, 1e-4 density
+    noise), with the reference hybrid c2p, the bracketed solver and the
+    learned guess"""
+    import baum_amd as B
+    n, L = 128, 16.0
+    h = 2 * L / n
+    ns = n + 2
+    x = -L - 0.5 * h + h * np.arange(ns)
+    K, G, rhoc = 100.0, 2.0, 1.28e-3
+    star = tov_isotropic(rhoc, K, G)
+    star["alpha"][0], star["psi4"][0] = star["alpha"][1], star["psi4"][1]
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    rb = np.sqrt(X * X + Y * Y + Z * Z).reshape(-1)
+    del X, Y, Z
+    inside = rb < star["rbs"]
+    M = star["M"]
+    psi4 = np.where(inside, np.interp(rb, star["rb"], star["psi4"]), (1 + M / (2 * np.maximum(rb, 1e-9))) ** 4)
+    alpha = np.where(inside, np.interp(rb, star["rb"], star["alpha"]),
+                     (1 - M / (2 * np.maximum(rb, 1e-9))) / (1 + M / (2 * np.maximum(rb, 1e-9))))
+    rhoatm = 1e-10 * rhoc
+    rho0 = np.where(inside, np.interp(rb, star["rb"], star["rho"]), rhoatm)
+    rho0 = np.maximum(rho0, rhoatm)
+    rng = np.random.default_rng(20261202)
+    rho = rho0 * (1 + 1e-4 * rng.standard_normal(rho0.shape))
+    eps = K * rho ** (G - 1) / (G - 1)
+    sg = psi4 ** 1.5
+    npts = ns ** 3
+    UZ, UQ = list(range(25)), list(range(25, 30))
+    RZ, RQ = list(range(30, 55)), list(range(55, 60))
+    ADMG, ADMK, PRIM, MADM = list(range(60, 66)), list(range(66, 72)), list(range(72, 80)), list(range(80, 91))
+    lev = B.Level([B.make_box(ns, ns, ns, x[0], x[0], x[0], h)], 91)
+    lev.enable(range(91))
+    for v in range(91):
+        lev.fill(v, 0.0)
+    for v in (0, 3, 5):
+        lev.fill(v, 1.0)
+    lev.put(6, 1.0 / psi4)
+    lev.put(18, alpha)
+    cons = np.array([sg * rho, sg * rho * eps])       # static fluid: D, tau; S_i = 0
+    z4 = B.z4c_params()
+    P = B.c2p_params(eos="ideal", gamma=G, K=K, rhoatm=rhoatm, errormax=1e-12, countmax=100)
+    hp = B.hrsc_params(eos="ideal", gamma=G, K=K)
+    status, iters = B.DeviceArray((npts,), np.int32), B.DeviceArray((npts,), np.int32)
+    fb = B.DeviceArray.from_host(np.zeros(1, np.int32))
+    out = {"grid": f"{n}^3 interior + 1 boundary layer, h = {h}", "tov": {"M": star["M"], "R_schwarzschild": star["R"],
+                                                                         "R_isotropic": star["rbs"]}}
+
+    def reset():
+        lev.put(UQ[0], cons[0])
+        lev.put(UQ[1], cons[1])
+        for v in UQ[2:]:
+            lev.fill(v, 0.0)
+        lev.put(PRIM[0], rho0)
+        lev.put(PRIM[5], K * rho0 ** G)
+        fb.put(np.zeros(1, np.int32))
+
+    res_p = {}
+    for name, method in (("hybrid", B.C2P_HYBRID), ("bracketed", B.C2P_BRACKETED), ("learned", B.C2P_LEARNED)):
+        kw = dict(method=method, mlp=net if method == B.C2P_LEARNED else None, fallbacks=fb)
+        reset()
+        lev.z4c_grhd_rhs(z4, P, hp, RZ + RQ, UZ + UQ, ADMG, ADMK, PRIM, MADM, status, iters, **kw)   # tunes
+        reps, tot, c2p_ms = 3, [], []
+        for _ in range(reps):
+            reset()
+            B.synchronize()
+            B.timer_reset()
+            B.timer_enable(True)
+            t0 = time.perf_counter()
+            lev.z4c_grhd_rhs(z4, P, hp, RZ + RQ, UZ + UQ, ADMG, ADMK, PRIM, MADM, status, iters, **kw)
+            B.synchronize()
+            tot.append(time.perf_counter() - t0)
+            B.timer_enable(False)
+            c2p_ms.append(B.timer_get("c2p")[0])
+        st, it = status.get(), iters.get()
+        res_p[name] = (lev.get(PRIM[5]), st)
+        out[name] = dict(rhs_ms=1e3 * min(tot), c2p_ms=min(c2p_ms),
+                         status_hist=np.bincount(st, minlength=5).tolist(), mean_iters=float(it.mean()),
+                         max_iters=int(it.max()))
+        if method == B.C2P_LEARNED:
+            out[name]["fallbacks"] = int(fb.get()[0])
+    # where the reference's solver converged (not its atmosphere or cold branches)
+    ok = res_p["hybrid"][1] == B.C2P_OK
+    out["hybrid"]["converged_points"] = int(ok.sum())
+    for name in ("bracketed", "learned"):
+        out[name]["max_rel_diff_p_vs_hybrid_where_converged"] = float(
+            np.max(np.abs(res_p[name][0][ok] / res_p["hybrid"][0][ok] - 1)))
+    out["c2p_learned_vs_hybrid_speedup"] = out["hybrid"]["c2p_ms"] / out["learned"]["c2p_ms"]
+    for a in (status, iters, fb):
+        a.free()
+    lev.free()
     return out
 
 
@@ -456,8 +587,13 @@ def main():
                          "ham_mom_l2_after": box["ham_mom_l2_after"]}
     # tile kernel chosen by the autotuner for each tuned box set, with the time of every variant
     out["rhs_tuning"] = B.tune_report().splitlines()
-    if "c2p" in only:
-        out["c2p"] = run_c2p(args)
+    if "c2p" in only or "c5" in only:
+        c2p_out, net = run_c2p(args)
+        if "c2p" in only:
+            out["c2p"] = c2p_out
+        if "c5" in only:
+            out["c5"] = run_c5(args, net)
+        net.free()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))
