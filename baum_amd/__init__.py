@@ -340,18 +340,35 @@ class Level:
         _sig(L, "bamhip_z4c_to_adm", C.c_int, [C.c_void_p, IP, C.c_double, IP, IP])
         check(L.bamhip_z4c_to_adm(self.h, _iarr(u), chipsipower, _iarr(adm_g), _iarr(adm_K)), "bamhip_z4c_to_adm")
 
+    @staticmethod
+    def _z4c_grhd(z4, c2p, hrsc, adm_g, adm_K, prim, madm, status, iters, method, mlp, polish_max, fallbacks, pgh):
+        keep = [_iarr(x) for x in (adm_g, adm_K, prim, madm)]
+        S = Z4cGrhdStruct(C.pointer(z4), C.pointer(c2p), C.pointer(hrsc), method, mlp.h if mlp is not None else None,
+                          polish_max, fallbacks.ptr if fallbacks is not None else None, pgh,
+                          *[C.cast(k, IP) for k in keep], status.ptr, iters.ptr)
+        return S, keep
+
     def z4c_grhd_rhs(self, z4, c2p, hrsc, unew, ucur, adm_g, adm_K, prim, madm, status, iters, upre=None, c=0.0,
                      method=0, mlp=None, polish_max=8, fallbacks=None, pgh=0):
         """evolve_rhs of the coupled Z4c + GRHD system; lists of 30 indices
         (25 Z4c fields, then D, tau, S_i); status / iters: DeviceArray int32"""
         L = lib()
         _sig(L, "bamhip_z4c_grhd_rhs", C.c_int, [C.c_void_p, C.POINTER(Z4cGrhdStruct), IP, IP, C.c_double, IP])
-        keep = [_iarr(x) for x in (adm_g, adm_K, prim, madm)]
-        S = Z4cGrhdStruct(C.pointer(z4), C.pointer(c2p), C.pointer(hrsc), method, mlp.h if mlp is not None else None,
-                          polish_max, fallbacks.ptr if fallbacks is not None else None, pgh,
-                          *[C.cast(k, IP) for k in keep], status.ptr, iters.ptr)
+        S, keep = self._z4c_grhd(z4, c2p, hrsc, adm_g, adm_K, prim, madm, status, iters, method, mlp, polish_max,
+                                 fallbacks, pgh)
         check(L.bamhip_z4c_grhd_rhs(self.h, C.byref(S), _iarr(unew), _iarr(upre) if upre is not None else None, c,
                                     _iarr(ucur)), "bamhip_z4c_grhd_rhs")
+
+    def z4c_grhd_rk4_step(self, z4, c2p, hrsc, dt, uc, up, r, v, adm_g, adm_K, prim, madm, status, iters,
+                          method=0, mlp=None, polish_max=8, fallbacks=None, pgh=0, algcon=True):
+        """one RK4 step of the coupled system (lists of 30 indices)"""
+        L = lib()
+        _sig(L, "bamhip_z4c_grhd_rk4_step", C.c_int,
+             [C.c_void_p, C.POINTER(Z4cGrhdStruct), C.c_double, IP, IP, IP, IP, C.c_int])
+        S, keep = self._z4c_grhd(z4, c2p, hrsc, adm_g, adm_K, prim, madm, status, iters, method, mlp, polish_max,
+                                 fallbacks, pgh)
+        check(L.bamhip_z4c_grhd_rk4_step(self.h, C.byref(S), dt, _iarr(uc), _iarr(up), _iarr(r), _iarr(v),
+                                         1 if algcon else 0), "bamhip_z4c_grhd_rk4_step")
 
     def z4c_algcon(self, g, A, subtractA=1, normalizedetg=1):
         check(lib().bamhip_z4c_algcon(self.h, _iarr(g), _iarr(A), subtractA, normalizedetg), "bamhip_z4c_algcon")

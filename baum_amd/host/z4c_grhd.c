@@ -52,3 +52,35 @@ int bamhip_z4c_grhd_rhs(bamhip_level *lev, const bamhip_z4c_grhd *S, const int *
   if ((e = bamhip_z4c_matter_source(lev, S->z4, unew, c, ucur, S->madm))) return e;
   return bamhip_z4c_set_boundary(lev, S->z4, unew, upre ? upre : unew, c, ucur, S->pgh);
 }
+
+/* One RK4 step of the coupled system: evolve_rk_rk4 (src/utility/evolve/
+ * rk.c:220-257) over the 30 evolved fields with the RHS above (c = 0 per
+ * stage), then z4_algcon_wrapper on the Z4c fields (z4.c:49-71).  The RHS's
+ * non-finite check is deferred to the end of the step, as in
+ * bamhip_z4c_rk4_step.  Lists hold 30 indices (25 Z4c, then D, tau, S_i). */
+int bamhip_z4c_grhd_rk4_step(bamhip_level *lev, const bamhip_z4c_grhd *S, double dt, const int *uc, const int *up,
+                             const int *r, const int *v, int algcon)
+{
+  const int nz = BAMHIP_Z4C_NVARS;
+  const double wc[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
+  const double wv[4] = {dt / 2, dt / 2, dt, 0.0};
+  int e, s, flag = 0;
+  if (!lev || !S || !uc || !up || !r || !v) return BAMHIP_EINVAL;
+  if ((e = bamhip_level_set_check(lev, 0))) return e;
+  if ((e = bamhip_level_nonfinite(lev, 1, 0))) goto out;
+  /* u_p = u_c */
+  if ((e = bamhip_vladd(lev, nz, up, 1.0, uc, 0.0, 0))) goto out;
+  if ((e = bamhip_vladd(lev, 5, up + nz, 1.0, uc + nz, 0.0, 0))) goto out;
+  for (s = 0; s < 4; s++) {
+    const int *src = (s == 0) ? uc : v;
+    if ((e = bamhip_z4c_grhd_rhs(lev, S, r, 0, 0.0, src))) goto out;
+    if ((e = bamhip_rk4_stage_update(lev, nz, uc, up, r, v, wc[s], wv[s]))) goto out;
+    if ((e = bamhip_rk4_stage_update(lev, 5, uc + nz, up + nz, r + nz, v + nz, wc[s], wv[s]))) goto out;
+  }
+  if (algcon && (e = bamhip_z4c_algcon_list(lev, S->z4, uc))) goto out;
+  if ((e = bamhip_level_nonfinite(lev, 0, &flag))) goto out;
+  if (flag) e = BAMHIP_ENONFINITE;
+out:
+  bamhip_level_set_check(lev, 1);
+  return e;
+}

@@ -492,6 +492,11 @@ typedef struct {
 } bamhip_z4c_grhd;
 int bamhip_z4c_grhd_rhs(bamhip_level *lev, const bamhip_z4c_grhd *S, const int *unew, const int *upre, double c,
                         const int *ucur);
+/* One RK4 step of the coupled system: evolve_rk_rk4 (src/utility/evolve/
+ * rk.c:220-257) over the 30 fields with bamhip_z4c_grhd_rhs per stage, then
+ * algcon on the Z4c fields (z4.c:49-71); lists of 30 indices as above. */
+int bamhip_z4c_grhd_rk4_step(bamhip_level *lev, const bamhip_z4c_grhd *S, double dt, const int *uc, const int *up,
+                             const int *r, const int *v, int algcon);
 
 /* ---------------------------------------------------------- punctures */
 /* Replaces track_moving_puncture_integrate (src/projects/Gauge/

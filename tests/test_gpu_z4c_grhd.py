@@ -114,3 +114,77 @@ def test_coupled_rhs_matches_composed_oracle():
     close(got_madm, madm, 1e-12, "matter ADM")
     close(got_rq, rq, 1e-9, "matter rhs")
     assert np.abs(rq).max() > 0 and np.abs(madm[0]).max() > 0
+
+
+def _oracle_rhs(box, oz4, oc2p, ohp, u30, pguess, rhoguess):
+    """the coupled RHS composed from the oracle (evolve_rhs order); returns
+    the 30 right-hand sides and the primitives"""
+    u = np.ascontiguousarray(u30[:25])
+    rz, nbad = O.z4c_rhs(box, oz4, u)
+    assert nbad == 0
+    O.z4c_radiative_bc(box, oz4, rz, u)
+    gadm, kadm = O.z4c_to_adm(box, u)
+    prim, st, it, _ = O.c2p(oc2p, u30[25:].reshape(5, -1), gadm.reshape(6, -1), pguess.reshape(-1),
+                            rhoguess=rhoguess.reshape(-1), method="hybrid")
+    assert np.all(st == B.C2P_OK)
+    prim = prim.reshape((8,) + u.shape[1:])
+    metric = np.concatenate([u[18:22], gadm])
+    rq = np.zeros((5,) + u.shape[1:])
+    madm = np.zeros((11,) + u.shape[1:])
+    O.grhd_sources(box, ohp, metric, kadm, prim[:6], rq, madm, rhoatm=1.28e-13)
+    O.grhd_fluxes(box, ohp, metric, prim[:5], rq)
+    O.z4c_matter_source(box, oz4, u, madm, rz)
+    return np.concatenate([rz, rq]), prim
+
+
+def test_coupled_rk4_step_matches_composed_oracle():
+    """evolve_rk_rk4 over the 30 fields (u_p = u_c; four stages of RHS, u_c +=
+    w_c r, v = u_p + w_v r; algcon), the c2p of each stage starting from the
+    previous stage's primitives, against the same composition of the oracle"""
+    z4, oz4 = B.z4c_params(), O.z4c_params()
+    c2p = B.c2p_params(eos="ideal", gamma=2.0, K=100.0, rhoatm=1.28e-13, errormax=1e-12)
+    oc2p = O.c2p_params(eos="ideal", gamma=2.0, K=100.0, rhoatm=1.28e-13, errormax=1e-12)
+    hp, ohp = B.hrsc_params(), O.hrsc_params()
+    box = O.make_box(N, N, N, X0, X0, X0, H)
+    u, cons, rho, p = _data(box, oc2p)
+    dt = 0.25 * H
+    u30 = np.concatenate([u, cons])
+    # oracle
+    uc, up = u30.copy(), u30.copy()
+    pg, rg = p.copy(), rho.copy()
+    wc, wv = [dt / 6, dt / 3, dt / 3, dt / 6], [dt / 2, dt / 2, dt, 0.0]
+    v = uc
+    for s in range(4):
+        r, prim = _oracle_rhs(box, oz4, oc2p, ohp, v if s else uc, pg, rg)
+        pg, rg = prim[5], prim[0]
+        uc = uc + wc[s] * r
+        v = up + wv[s] * r
+    O.z4c_algcon(box, uc[:25])
+    # device
+    UC, UP, R, V = (list(range(30 * k, 30 * k + 30)) for k in range(4))
+    ADMG2, ADMK2 = list(range(120, 126)), list(range(126, 132))
+    PRIM2, MADM2 = list(range(132, 140)), list(range(140, 151))
+    lev = B.Level([B.make_box(N, N, N, X0, X0, X0, H)], 151)
+    lev.enable(range(151))
+    for k in range(151):
+        lev.fill(k, 0.0)
+    lev.put_state(UC, u30.reshape(30, -1))
+    lev.put(PRIM2[0], rho.reshape(-1))
+    lev.put(PRIM2[5], p.reshape(-1))
+    status = B.DeviceArray((N ** 3,), np.int32)
+    iters = B.DeviceArray((N ** 3,), np.int32)
+    lev.z4c_grhd_rk4_step(z4, c2p, hp, dt, UC, UP, R, V, ADMG2, ADMK2, PRIM2, MADM2, status, iters,
+                          method=B.C2P_HYBRID)
+    got = lev.get_state(UC).reshape(uc.shape)
+    st = status.get()
+    status.free()
+    iters.free()
+    lev.free()
+    assert np.all(st == B.C2P_OK)
+    for f in range(30):
+        change = np.abs(uc[f] - u30[f]).max()
+        if change == 0:                  # B^i without withB
+            np.testing.assert_array_equal(got[f], uc[f])
+            continue
+        err = np.abs(got[f] - uc[f]).max()
+        assert err <= 1e-9 * change + 1e-15 * np.abs(uc[f]).max(), (f, err, change)
