@@ -410,10 +410,10 @@ def tov_isotropic(rhoc, K=100.0, G=2.0, dr=2e-3):
 
 
 def run_c5(args, net):
-    """one evaluation of the coupled Z4c + GRHD right-hand side on a 128^3
-    box holding a TOV star (Gamma 2, K 100, rho_c 1.28e-3, 1e-4 density
-    noise), with the reference hybrid c2p, the bracketed solver and the
-    learned guess"""
+    """the coupled Z4c + GRHD system on a 128^3 box holding a TOV star
+    (Gamma 2, K 100, rho_c 1.28e-3, 1e-4 density noise): one right-hand side
+    evaluation and one RK4 step, each with the reference hybrid c2p, the
+    bracketed solver and the learned guess"""
     import baum_amd as B
     n, L = 128, 16.0
     h = 2 * L / n
@@ -441,14 +441,19 @@ def run_c5(args, net):
     UZ, UQ = list(range(25)), list(range(25, 30))
     RZ, RQ = list(range(30, 55)), list(range(55, 60))
     ADMG, ADMK, PRIM, MADM = list(range(60, 66)), list(range(66, 72)), list(range(72, 80)), list(range(80, 91))
-    lev = B.Level([B.make_box(ns, ns, ns, x[0], x[0], x[0], h)], 91)
-    lev.enable(range(91))
-    for v in range(91):
+    UP, V = list(range(91, 121)), list(range(121, 151))
+    lev = B.Level([B.make_box(ns, ns, ns, x[0], x[0], x[0], h)], 151)
+    lev.enable(range(151))
+    for v in range(151):
         lev.fill(v, 0.0)
-    for v in (0, 3, 5):
-        lev.fill(v, 1.0)
-    lev.put(6, 1.0 / psi4)
-    lev.put(18, alpha)
+
+    def reset_geometry():
+        for v in range(25):
+            lev.fill(v, 1.0 if v in (0, 3, 5) else 0.0)
+        lev.put(6, 1.0 / psi4)
+        lev.put(18, alpha)
+
+    reset_geometry()
     cons = np.array([sg * rho, sg * rho * eps])       # static fluid: D, tau; S_i = 0
     z4 = B.z4c_params()
     P = B.c2p_params(eos="ideal", gamma=G, K=K, rhoatm=rhoatm, errormax=1e-12, countmax=100)
@@ -491,6 +496,29 @@ def run_c5(args, net):
                          max_iters=int(it.max()))
         if method == B.C2P_LEARNED:
             out[name]["fallbacks"] = int(fb.get()[0])
+    # one RK4 step of the coupled system (four RHS evaluations, four c2p)
+    dt = 0.25 * h
+    for name, method in (("hybrid", B.C2P_HYBRID), ("bracketed", B.C2P_BRACKETED), ("learned", B.C2P_LEARNED)):
+        kw = dict(method=method, mlp=net if method == B.C2P_LEARNED else None, fallbacks=fb)
+        tot, c2p_ms = [], []
+        for rep in range(3):
+            reset_geometry()
+            reset()
+            B.synchronize()
+            B.timer_reset()
+            B.timer_enable(True)
+            t0 = time.perf_counter()
+            lev.z4c_grhd_rk4_step(z4, P, hp, dt, UZ + UQ, UP, RZ + RQ, V, ADMG, ADMK, PRIM, MADM, status, iters, **kw)
+            B.synchronize()
+            if rep:                      # the first step is a warm-up
+                tot.append(time.perf_counter() - t0)
+                c2p_ms.append(B.timer_get("c2p")[0])
+            B.timer_enable(False)
+        out[name].update(step_ms=1e3 * min(tot), step_c2p_ms=min(c2p_ms), step_updates_per_s=n ** 3 / min(tot),
+                         step_status_hist=np.bincount(status.get(), minlength=5).tolist())
+        if method == B.C2P_LEARNED:
+            out[name]["step_fallbacks"] = int(fb.get()[0])
+    out["step_c2p_learned_vs_hybrid_speedup"] = out["hybrid"]["step_c2p_ms"] / out["learned"]["step_c2p_ms"]
     # where the reference's solver converged (not its atmosphere or cold branches)
     ok = res_p["hybrid"][1] == B.C2P_OK
     out["hybrid"]["converged_points"] = int(ok.sum())
@@ -543,6 +571,11 @@ def traffic_from_profile():
     return None
 
 
+def progress(msg):
+    """one line on stderr as each workload starts and ends (stdout carries only the JSON line)"""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -557,7 +590,9 @@ def main():
     B.set_xcd_map(args.xcd_map != 0)
     out = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
+    progress("fmr hierarchy")
     fmr = run_fmr(args) if "fmr" in only else None
+    progress("192^3 box")
     box = run_box(args) if "box192" in only else None
     if fmr:
         out.update(value=fmr["value"], unit="grid-point-updates/s", ms_per_step=fmr["ms_per_step"])
@@ -588,14 +623,18 @@ def main():
     # tile kernel chosen by the autotuner for each tuned box set, with the time of every variant
     out["rhs_tuning"] = B.tune_report().splitlines()
     if "c2p" in only or "c5" in only:
+        progress("con2prim on 2^24 points, network training")
         c2p_out, net = run_c2p(args)
         if "c2p" in only:
             out["c2p"] = c2p_out
         if "c5" in only:
+            progress("coupled Z4c + GRHD on the TOV star")
             out["c5"] = run_c5(args, net)
         net.free()
     if not args.no_cpu:
+        progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    progress("done")
     print(json.dumps(out))
 
 
