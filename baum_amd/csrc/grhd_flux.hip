@@ -32,9 +32,10 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kReach = 3;  // stencil points on each side of a segment
 
-// tile of a direction: T points along the line, L lines across
+// tile of a direction: T points along the line, L lines across; the T + 1
+// interfaces of the L lines are one per thread
 template <int DIR> struct FluxTile {
-  static constexpr int T = DIR == 0 ? 64 : 16;
+  static constexpr int T = DIR == 0 ? 63 : 15;
   static constexpr int L = DIR == 0 ? 4 : 16;
 };
 
@@ -216,7 +217,7 @@ __global__ void __launch_bounds__(kBlock) grhd_flux_kernel(FluxArgs A) {
   // one point per thread: rhs -= (F[p + 1] - F[p]) / h  (times c when c != 0)
   const int pl = DIR == 0 ? tid % T : tid / L, line = DIR == 0 ? tid / T : tid % L;
   const int p = s + pl, c = c0 + line;
-  if (p >= A.N || c >= A.na) return;
+  if (pl >= T || line >= L || p >= A.N || c >= A.na) return;  // 256 threads, T * L points
   const long ijk = base + A.sa * p + A.sc * c;
   for (int k = 0; k < 5; k++) {
     const double dF = A.ooh * (flx[k][line][pl + 1] - flx[k][line][pl]);
@@ -229,8 +230,8 @@ __global__ void __launch_bounds__(kBlock) grhd_flux_kernel(FluxArgs A) {
   }
 }
 
-static_assert(FluxTile<0>::T * FluxTile<0>::L == kBlock && FluxTile<1>::T * FluxTile<1>::L == kBlock,
-              "one point of the tile per thread");
+static_assert((FluxTile<0>::T + 1) * FluxTile<0>::L == kBlock && (FluxTile<1>::T + 1) * FluxTile<1>::L == kBlock,
+              "one interface of the tile per thread");
 
 constexpr int kVperm[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};                                   // dir_perm_b
 constexpr int kTperm[3][6] = {{0, 1, 2, 3, 4, 5}, {3, 4, 1, 5, 2, 0}, {5, 2, 4, 0, 1, 3}};  // dir_perm, tensors
