@@ -6,7 +6,8 @@
 // 0 (bam_grhd.c:51-52).  At every inner point of every box:
 //   - detg <= 0 or alpha <= 0: no source and matter ADM variables 0;
 //   - centred first derivatives of alpha, beta^i, g_ij, 2nd order at distance
-//     1 from a box face (or with order_centered 2), else 4th order;
+//     1 from a box face (or with order_centered 2), else 4th order, in the
+//     multiplied form of the generated code;
 //   - T^00, T^0i, T^ij, T^0_j from the primitives, W capped at grhd_Wlor_max
 //     when v^2 >= 1;
 //   - s_tau = alpha sqrt(g) (T^00 (beta beta K - beta d alpha)
@@ -17,10 +18,14 @@
 //   - no momentum source below the atmosphere density, no tau source for a
 //     cold EOS, all 16 values 0 when any is not finite;
 //   - unew += source (times c when c != 0).
-// One thread per point; the neighbours of the derivative stencils come
-// through L1/L2.  The sums run over full 3x3 index ranges in a fixed order
-// and FP contraction is off, so the results are bitwise those of the C
-// restatement that the tests compare with.
+// A block computes a 16x4x4 tile of points.  The ten metric fields pass
+// through LDS one at a time (the tile plus 2 points on every side, indices
+// clamped to the box: only the stencils of inner points read the halo, and
+// they stay inside it), two fields ahead prefetched into registers while the
+// current one is differentiated; each thread keeps its 30 first derivatives
+// in registers and then forms the sources.  The sums run over full 3x3 index
+// ranges in a fixed order and FP contraction is off, so the results are
+// bitwise those of the C restatement that the tests compare with.
 #pragma clang fp contract(off)
 
 #include <cmath>
@@ -31,10 +36,14 @@ namespace bamhip {
 namespace {
 
 constexpr int kSrcBlock = 256;
+constexpr int kTX = 16, kTY = 4, kTZ = 4, kHalo = 2;
+constexpr int kSX = kTX + 2 * kHalo, kSY = kTY + 2 * kHalo, kSZ = kTZ + 2 * kHalo, kSN = kSX * kSY * kSZ;
+constexpr int kPer = (kSN + kSrcBlock - 1) / kSrcBlock;  // staged values per thread and field
+static_assert(kTX * kTY * kTZ == kSrcBlock, "one point of the tile per thread");
 
 struct SrcArgs {
   int m, n, o, skip[6];
-  double h[3];
+  double oo2h[3];  // 1 / (2 h)
   double rhoatm, wlormax, c;
   int cold, order_centered;
   const double *met[10], *K[6], *prim[6];
@@ -43,32 +52,85 @@ struct SrcArgs {
 
 __device__ constexpr int s6(int a, int b) { return a <= b ? (a == 0 ? b : (a == 1 ? 2 + b : 5)) : s6(b, a); }
 
-__device__ __forceinline__ double d1(const double *u, long q, long s, double h, int ord) {
-  if (ord == 2) return (u[q + s] - u[q - s]) / (2. * h);
-  return (u[q - 2 * s] - 8. * u[q - s] + 8. * u[q + s] - u[q + 2 * s]) / (12. * h);
+// first derivative, centred, in the form of the generated code
+// (grhd_sources.c: oo2dx (-u[-1] + u[1]), 1/6 oo2dx (u[-2] + 8 (-u[-1] + u[1]) - u[2]))
+__device__ __forceinline__ double d1(const double *u, int q, int s, double oo2h, int ord) {
+  if (ord == 2) return oo2h * (-u[q - s] + u[q + s]);
+  return 0.16666666666666666667 * oo2h * (u[q - 2 * s] + 8. * (-u[q - s] + u[q + s]) - u[q + 2 * s]);
+}
+
+// the staged values of a field for this thread: off[e] is the box index of
+// staged element tid + e * 256 (clamped to the box), -1 past the tile
+__device__ __forceinline__ void stage_load(const double *f, const long off[kPer], double r[kPer]) {
+#pragma unroll
+  for (int e = 0; e < kPer; e++) r[e] = off[e] >= 0 ? f[off[e]] : 0.;
 }
 
 __global__ void __launch_bounds__(kSrcBlock) grhd_source_kernel(SrcArgs A) {
-  const long npts = (long)A.m * A.n * A.o;
-  const long q = (long)blockIdx.x * kSrcBlock + threadIdx.x;
-  if (q >= npts) return;
-  const long m = A.m, mn = (long)A.m * A.n;
-  const int i = (int)(q % m), j = (int)((q / m) % A.n), k = (int)(q / mn);
+  __shared__ double buf[3][kSN];
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * kTX, j0 = blockIdx.y * kTY, k0 = blockIdx.z * kTZ;
+  const int i = i0 + tid % kTX, j = j0 + (tid / kTX) % kTY, k = k0 + tid / (kTX * kTY);
   const int imax = A.m - 1, jmax = A.n - 1, kmax = A.o - 1;
-  if (i < A.skip[0] || i > imax - A.skip[1] || j < A.skip[2] || j > jmax - A.skip[3] || k < A.skip[4] ||
-      k > kmax - A.skip[5])
-    return;
-  const long str[3] = {1, m, mn};
-  const double al = A.met[0][q];
-  const double be[3] = {A.met[1][q], A.met[2][q], A.met[3][q]};
-  double g[6], K[6];
+  const bool inner = i >= A.skip[0] && i <= imax - A.skip[1] && j >= A.skip[2] && j <= jmax - A.skip[3] &&
+                     k >= A.skip[4] && k <= kmax - A.skip[5];
+  const int ord =
+      (A.order_centered == 2 || i == 1 || j == 1 || k == 1 || i == imax - 1 || j == jmax - 1 || k == kmax - 1) ? 2 : 4;
+  // position of the point in the staged tile and the staged strides
+  const int c = (tid % kTX + kHalo) + kSX * ((tid / kTX) % kTY + kHalo) + kSX * kSY * (tid / (kTX * kTY) + kHalo);
+  const int sst[3] = {1, kSX, kSX * kSY};
+  long off[kPer];
 #pragma unroll
-  for (int a = 0; a < 6; a++) {
-    g[a] = A.met[4 + a][q];
-    K[a] = A.K[a][q];
+  for (int e = 0; e < kPer; e++) {
+    const int idx = tid + e * kSrcBlock;
+    const int sx = idx % kSX, sy = (idx / kSX) % kSY, sz = idx / (kSX * kSY);
+    const int gi = min(max(i0 - kHalo + sx, 0), A.m - 1), gj = min(max(j0 - kHalo + sy, 0), A.n - 1),
+              gk = min(max(k0 - kHalo + sz, 0), A.o - 1);
+    off[e] = idx < kSN ? gi + (long)A.m * gj + (long)A.m * A.n * gk : -1;
   }
-  const double rho = A.prim[0][q], eps = A.prim[1][q], p = A.prim[5][q];
-  const double vu[3] = {A.prim[2][q], A.prim[3][q], A.prim[4][q]};
+  // point values read once, issued before the staging so their latency hides behind it
+  long q = 0;
+  double K[6], rho = 0., eps = 0., p = 0., vu[3] = {0., 0., 0.};
+  if (inner) {
+    q = (long)i + (long)A.m * j + (long)A.m * A.n * k;
+#pragma unroll
+    for (int a = 0; a < 6; a++) K[a] = A.K[a][q];
+    rho = A.prim[0][q], eps = A.prim[1][q], p = A.prim[5][q];
+    vu[0] = A.prim[2][q], vu[1] = A.prim[3][q], vu[2] = A.prim[4][q];
+  }
+  // the ten metric fields through three LDS buffers, two fields ahead: value
+  // at the point and its 3 derivatives
+  double val[10], der[3][10];
+  double r0[kPer], r1[kPer];
+  stage_load(A.met[0], off, r0);
+  stage_load(A.met[1], off, r1);
+#pragma unroll
+  for (int e = 0; e < kPer; e++)
+    if (tid + e * kSrcBlock < kSN) buf[0][tid + e * kSrcBlock] = r0[e];
+  __syncthreads();
+#pragma unroll
+  for (int f = 0; f < 10; f++) {
+    // r1 holds field f + 1; r0 receives field f + 2
+    if (f + 2 < 10) stage_load(A.met[f + 2], off, r0);
+    const double *u = buf[f % 3];
+    val[f] = u[c];
+#pragma unroll
+    for (int d = 0; d < 3; d++) der[d][f] = inner ? d1(u, c, sst[d], A.oo2h[d], ord) : 0.;
+    if (f + 1 < 10) {
+#pragma unroll
+      for (int e = 0; e < kPer; e++)
+        if (tid + e * kSrcBlock < kSN) buf[(f + 1) % 3][tid + e * kSrcBlock] = r1[e];
+    }
+#pragma unroll
+    for (int e = 0; e < kPer; e++) r1[e] = r0[e];
+    __syncthreads();
+  }
+  if (!inner) return;
+  const double al = val[0];
+  const double be[3] = {val[1], val[2], val[3]};
+  double g[6];
+#pragma unroll
+  for (int a = 0; a < 6; a++) g[a] = val[4 + a];
   const double adj[6] = {g[3] * g[5] - g[4] * g[4], g[2] * g[4] - g[1] * g[5], g[1] * g[4] - g[2] * g[3],
                          g[0] * g[5] - g[2] * g[2], g[1] * g[2] - g[0] * g[4], g[0] * g[3] - g[1] * g[1]};
   const double detg = g[0] * adj[0] + g[1] * adj[1] + g[2] * adj[2];
@@ -76,9 +138,6 @@ __global__ void __launch_bounds__(kSrcBlock) grhd_source_kernel(SrcArgs A) {
 #pragma unroll
   for (int a = 0; a < 11; a++) ad[a] = 0.;
   if (detg > 0. && al > 0.) {
-    const int ord =
-        (A.order_centered == 2 || i == 1 || j == 1 || k == 1 || i == imax - 1 || j == jmax - 1 || k == kmax - 1) ? 2
-                                                                                                                : 4;
     const double detginv = 1. / detg;
     double gi[6];
 #pragma unroll
@@ -112,7 +171,7 @@ __global__ void __launch_bounds__(kSrcBlock) grhd_source_kernel(SrcArgs A) {
       for (int b = a; b < 3; b++) Tij[s6(a, b)] = W2hrho * vs[a] * vs[b] + p * (gi[s6(a, b)] - be[a] * be[b] / al2);
     double dal[3];
 #pragma unroll
-    for (int d = 0; d < 3; d++) dal[d] = d1(A.met[0], q, str[d], A.h[d], ord);
+    for (int d = 0; d < 3; d++) dal[d] = der[d][0];
     double bbK = 0., bda = 0., t0 = 0., tk = 0.;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -127,16 +186,13 @@ __global__ void __launch_bounds__(kSrcBlock) grhd_source_kernel(SrcArgs A) {
       t0 += T0i[a] * (2. * bK - dal[a]);
     }
     sT = sqrtmdetg * (T00 * (bbK - bda) + t0 + tk);
-    // one direction at a time: its 9 derivatives feed its momentum source
-    // (the barrier keeps the next direction's stencil loads from being
-    // hoisted, which would hold 120 loads in registers at once)
 #pragma unroll
     for (int d = 0; d < 3; d++) {
       double dbe[3], dg[6];
 #pragma unroll
-      for (int a = 0; a < 3; a++) dbe[a] = d1(A.met[1 + a], q, str[d], A.h[d], ord);
+      for (int a = 0; a < 3; a++) dbe[a] = der[d][1 + a];
 #pragma unroll
-      for (int a = 0; a < 6; a++) dg[a] = d1(A.met[4 + a], q, str[d], A.h[d], ord);
+      for (int a = 0; a < 6; a++) dg[a] = der[d][4 + a];
       double bbdg = 0., t0dg = 0., tdg = 0., tdb = 0.;
 #pragma unroll
       for (int a = 0; a < 3; a++) {
@@ -149,7 +205,6 @@ __global__ void __launch_bounds__(kSrcBlock) grhd_source_kernel(SrcArgs A) {
         }
       }
       sS[d] = sqrtmdetg * (T00 * (0.5 * bbdg - al * dal[d]) + t0dg + tdb + 0.5 * tdg);
-      asm volatile("" ::: "memory");
     }
     double vv = 0.;
 #pragma unroll
@@ -216,15 +271,15 @@ extern "C" int bamhip_grhd_sources(bamhip_level *lev, const bamhip_hrsc_params *
     SrcArgs A{};
     A.m = B.m, A.n = B.n, A.o = B.o;
     for (int f = 0; f < 6; f++) A.skip[f] = B.skip[f];
-    A.h[0] = B.dx, A.h[1] = B.dy, A.h[2] = B.dz;
+    A.oo2h[0] = 1 / (2 * B.dx), A.oo2h[1] = 1 / (2 * B.dy), A.oo2h[2] = 1 / (2 * B.dz);
     A.rhoatm = rhoatm, A.wlormax = par->wlormax, A.c = c;
     A.cold = par->eos == 1, A.order_centered = order_centered;
     for (int a = 0; a < 10; a++) A.met[a] = met[a] + B.noffset;
     for (int a = 0; a < 6; a++) A.K[a] = kk[a] + B.noffset, A.prim[a] = pr[a] + B.noffset;
     for (int a = 0; a < 5; a++) A.rhs[a] = rhs[a] + B.noffset;
     for (int a = 0; a < 11; a++) A.madm[a] = ad[a] + B.noffset;
-    const long npts = (long)B.m * B.n * B.o;
-    grhd_source_kernel<<<(unsigned)((npts + kSrcBlock - 1) / kSrcBlock), kSrcBlock, 0, stream()>>>(A);
+    dim3 grid((B.m + kTX - 1) / kTX, (B.n + kTY - 1) / kTY, (B.o + kTZ - 1) / kTZ);
+    grhd_source_kernel<<<grid, kSrcBlock, 0, stream()>>>(A);
   }
   BH_HIP(hipGetLastError());
   return BAMHIP_OK;

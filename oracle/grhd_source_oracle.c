@@ -26,11 +26,13 @@
 
 static const int S6[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
 
-/* first derivative of u along stride s, centred, 2nd or 4th order */
-static double d1(const double *u, long q, long s, double h, int ord)
+/* first derivative of u along stride s, centred, 2nd or 4th order, in the
+ * form of the generated code (grhd_sources.c: oo2dx = 1/(2 dx),
+ * oo2dx (-u[-1] + u[1]) and 1/6 oo2dx (u[-2] + 8 (-u[-1] + u[1]) - u[2])) */
+static double d1(const double *u, long q, long s, double oo2h, int ord)
 {
-  if (ord == 2) return (u[q + s] - u[q - s]) / (2. * h);
-  return (u[q - 2 * s] - 8. * u[q - s] + 8. * u[q + s] - u[q + 2 * s]) / (12. * h);
+  if (ord == 2) return oo2h * (-u[q - s] + u[q + s]);
+  return 0.16666666666666666667 * oo2h * (u[q - 2 * s] + 8. * (-u[q - s] + u[q + s]) - u[q + 2 * s]);
 }
 
 void ob_grhd_sources(const ob_box *B, const ob_hrsc_params *P, double rhoatm, int order_centered, double c,
@@ -39,7 +41,7 @@ void ob_grhd_sources(const ob_box *B, const ob_hrsc_params *P, double rhoatm, in
 {
   const long m = B->m, mn = (long)B->m * B->n;
   const long str[3] = {1, m, mn};
-  const double h[3] = {B->dx, B->dy, B->dz};
+  const double oo2h[3] = {1 / (2 * B->dx), 1 / (2 * B->dy), 1 / (2 * B->dz)};
   const int imax = B->m - 1, jmax = B->n - 1, kmax = B->o - 1;
   for (int k = B->skip[4]; k <= kmax - B->skip[5]; k++)
     for (int j = B->skip[2]; j <= jmax - B->skip[3]; j++)
@@ -66,9 +68,9 @@ void ob_grhd_sources(const ob_box *B, const ob_hrsc_params *P, double rhoatm, in
                               : 4;
           double dal[3], dbe[3][3], dg[3][6];
           for (int d = 0; d < 3; d++) {
-            dal[d] = d1(metric[0], q, str[d], h[d], ord);
-            for (int a = 0; a < 3; a++) dbe[d][a] = d1(metric[1 + a], q, str[d], h[d], ord);
-            for (int a = 0; a < 6; a++) dg[d][a] = d1(metric[4 + a], q, str[d], h[d], ord);
+            dal[d] = d1(metric[0], q, str[d], oo2h[d], ord);
+            for (int a = 0; a < 3; a++) dbe[d][a] = d1(metric[1 + a], q, str[d], oo2h[d], ord);
+            for (int a = 0; a < 6; a++) dg[d][a] = d1(metric[4 + a], q, str[d], oo2h[d], ord);
           }
           const double detginv = 1. / detg;
           double gi[6];
