@@ -370,6 +370,33 @@ class Level:
         check(L.bamhip_z4c_grhd_rk4_step(self.h, C.byref(S), dt, _iarr(uc), _iarr(up), _iarr(r), _iarr(v),
                                          1 if algcon else 0), "bamhip_z4c_grhd_rk4_step")
 
+    def reduce(self, vars, kind):
+        """write_level_0d's reduction ("max", "min", "norm", "norminf", "integral") of each variable"""
+        L = lib()
+        _sig(L, "bamhip_level_reduce", C.c_int, [C.c_void_p, C.c_int, IP, C.c_int, DP])
+        out = (C.c_double * len(vars))()
+        check(L.bamhip_level_reduce(self.h, len(vars), _iarr(vars), OUTPUT_0D.index(kind), out),
+              "bamhip_level_reduce")
+        return np.array(out[:])
+
+    def output_0d(self, vars, names, kind, time, outdir):
+        """append one 0d line per variable in the reference's format"""
+        L = lib()
+        _sig(L, "bamhip_output_0d", C.c_int, [C.c_void_p, C.c_int, IP, C.POINTER(C.c_char_p), C.c_int, C.c_double,
+                                               C.c_char_p])
+        nm = (C.c_char_p * len(names))(*[n.encode() for n in names])
+        check(L.bamhip_output_0d(self.h, len(vars), _iarr(vars), nm, OUTPUT_0D.index(kind), time,
+                                 str(outdir).encode()), "bamhip_output_0d")
+
+    def output_1d(self, vars, names, line, time, outdir, origin=(0.0, 0.0, 0.0)):
+        """append one 1d block per variable ("x", "y" or "z" line) in the reference's format"""
+        L = lib()
+        _sig(L, "bamhip_output_1d", C.c_int, [C.c_void_p, C.c_int, IP, C.POINTER(C.c_char_p), C.c_int, C.c_double,
+                                               C.c_double, C.c_double, C.c_double, C.c_char_p])
+        nm = (C.c_char_p * len(names))(*[n.encode() for n in names])
+        check(L.bamhip_output_1d(self.h, len(vars), _iarr(vars), nm, "dxyz".index(line), *origin, time,
+                                 str(outdir).encode()), "bamhip_output_1d")
+
     def z4c_algcon(self, g, A, subtractA=1, normalizedetg=1):
         check(lib().bamhip_z4c_algcon(self.h, _iarr(g), _iarr(A), subtractA, normalizedetg), "bamhip_z4c_algcon")
 
@@ -523,6 +550,7 @@ def hrsc_params(eos="ideal", gamma=2.0, K=100.0, vmax=0.999, wlormax=1e5, rieman
 
 
 C2P_LEARNED_METHOD = 3
+OUTPUT_0D = ["max", "min", "norm", "norminf", "integral"]
 
 
 class Z4cGrhdStruct(C.Structure):

@@ -85,6 +85,17 @@ double *bamhip_level_ptr(bamhip_level *lev, int var);
 int bamhip_level_put(bamhip_level *lev, int var, const double *host);
 int bamhip_level_get(bamhip_level *lev, int var, double *host);
 int bamhip_level_fill(bamhip_level *lev, int var, double value);
+/* the level number and box count, and box b with its noffset (host copies) */
+int bamhip_level_info(const bamhip_level *lev, int *l, int *nboxes);
+int bamhip_level_box(const bamhip_level *lev, int b, bamhip_box *box, long *noffset);
+/* Reductions of write_level_0d (src/utility/output/output0d.c:28-118)
+ * through bampi_allreduce (src/main/bampi/reduce.c:36-131) over the inner
+ * points of the level: type 0 max, 1 min, 2 l2 norm (sum of squares over
+ * the count of all nv variables, times sqrt(nv)), 3 max |u|, 4 sum * dx^3.
+ * result[nv] on the host; deterministic (fixed blocks, fixed sum order). */
+int bamhip_level_reduce(bamhip_level *lev, int nv, const int *vars, int type, double *result);
+/* host out[t] = var[idx[t]] for n level indices (bufferorderedpoints) */
+int bamhip_level_gather(bamhip_level *lev, int var, long n, const long *idx, double *out);
 
 /* ------------------------------------------------------------------- Z4c */
 /* Snapshot of the parameters z4_rhs_movpunc_N reads on every call
@@ -513,6 +524,22 @@ int bamhip_z4c_grhd_rk4_step(bamhip_level *lev, const bamhip_z4c_grhd *S, double
 #define BAMHIP_PUNCTURE_PIN_Z 1
 int bamhip_puncture_track(bamhip_level *lev, const int *beta, const int *beta_p, double dt, int order, int flags,
                           int np, double *pos);
+
+/* --------------------------------------------------------------- output */
+/* write_level_0d (src/utility/output/output0d.c:28-118, write_scalar_N
+ * :11-25): for each variable the line "time  value  " (%22.15e) appended to
+ * <outdir>/<names[j]>_<max|min|norm|norminf|integral>.l<l> (output.c:219);
+ * type as bamhip_level_reduce. */
+int bamhip_output_0d(bamhip_level *lev, int nv, const int *vars, const char *const *names, int type, double time,
+                     const char *outdir);
+/* write_level_1d for the x, y, z lines (line 1, 2, 3) through the output
+ * origin (outputx0..z0) snapped to the next grid point, without
+ * interpolation (1doutinterpolate = no; output1d.c:18-103,300-420,
+ * points.c:278-335): "\"Time = t\"", rows "coordinate  value" (%22.15e, -1
+ * where no box holds the point) and a blank line, appended to
+ * <outdir>/<names[j]>.<x|y|z><l>. */
+int bamhip_output_1d(bamhip_level *lev, int nv, const int *vars, const char *const *names, int line, double x0,
+                     double y0, double z0, double time, const char *outdir);
 
 /* --------------------------------------------------------------- graphs */
 /* Capture the device work issued on the library stream between begin and

@@ -97,6 +97,11 @@ def lib():
         L.ob_z4c_matter_source.restype = None
         L.ob_z4c_matter_source.argtypes = [C.POINTER(Box), C.POINTER(Z4cParams), C.c_double, C.POINTER(DP),
                                            C.POINTER(DP), C.POINTER(DP)]
+        L.ob_reduce_0d.restype = None
+        L.ob_reduce_0d.argtypes = [C.c_int, C.POINTER(Box), C.POINTER(DP), C.c_int, C.c_int, DP]
+        L.ob_line_1d.restype = C.c_int
+        L.ob_line_1d.argtypes = [C.c_int, C.POINTER(Box), C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                 DP, IP, C.POINTER(C.c_long)]
         L.ob_interp_point.restype = C.c_double
         L.ob_interp_point.argtypes = [C.c_int, C.POINTER(Box), C.POINTER(DP), C.c_double, C.c_double, C.c_double,
                                       C.c_int, C.POINTER(C.c_int)]
@@ -477,3 +482,26 @@ def z4c_to_adm(box, u, chipsipower=-4.0):
     k = np.zeros((6,) + u.shape[1:])
     lib().ob_z4c_to_adm(C.byref(box), chipsipower, _ptrs(u), _ptrs(g), _ptrs(k))
     return g, k
+
+
+# ---------------------------------------------------------------- 0d / 1d output
+REDUCE_TYPES = {"max": 0, "min": 1, "norm": 2, "norminf": 3, "integral": 4}
+
+
+def reduce_0d(boxes, fields, kind):
+    """write_level_0d's reduction of each variable: fields is a list over
+    variables of lists over boxes"""
+    ptrs, keep = _box_ptrs(fields)
+    out = np.zeros(len(fields))
+    lib().ob_reduce_0d(len(boxes), _boxes(boxes), ptrs, len(fields), REDUCE_TYPES[kind], out.ctypes.data_as(DP))
+    return out
+
+
+def line_1d(boxes, line, origin=(0.0, 0.0, 0.0), maxp=100000):
+    """the 1d output line ("x", "y", "z"): (coordinates, box of each point or -1, index in that box)"""
+    crd = np.zeros(maxp)
+    bo = np.zeros(maxp, dtype=np.int32)
+    ix = np.zeros(maxp, dtype=np.int64)
+    n = lib().ob_line_1d(len(boxes), _boxes(boxes), {"x": 1, "y": 2, "z": 3}[line], *origin, maxp,
+                         crd.ctypes.data_as(DP), bo.ctypes.data_as(IP), ix.ctypes.data_as(C.POINTER(C.c_long)))
+    return crd[:n], bo[:n], ix[:n]

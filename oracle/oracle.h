@@ -194,6 +194,14 @@ void ob_grhd_sources(const ob_box *B, const ob_hrsc_params *P, double rhoatm, in
 void ob_z4c_matter_source(const ob_box *B, const ob_z4c_params *P, double c, const double *const *u,
                           const double *const *madm, double *const *r);
 
+/* 0d reductions and 1d point lists (output_oracle.c).  var[v * nb + b]: the
+ * array of variable v on box b (nv <= 64).  ob_line_1d: the points of the x,
+ * y or z line (line 1..3), their coordinate along the line, the box holding
+ * each (-1: none) and the index in that box; returns the point count. */
+void ob_reduce_0d(int nb, const ob_box *boxes, const double *const *var, int nv, int type, double *result);
+int ob_line_1d(int nb, const ob_box *boxes, int line, double x0, double y0, double z0, int maxp, double *coord,
+               int *box_of, long *index);
+
 /* moving-puncture tracking (puncture_oracle.c): interpolate_xyz_scalar on a
  * level of nb boxes (var[b] = the variable on box b; found = 0 and value 0
  * when no box holds the interpolation cube), and one step of
