@@ -334,6 +334,22 @@ class Level:
         check(L.bamhip_z4c_matter_source(self.h, C.byref(par), _iarr(unew), c, _iarr(ucur), _iarr(madm)),
               "bamhip_z4c_matter_source")
 
+    def punctures_bl(self, masses, positions, adm_g, adm_K):
+        """Brill-Lindquist ADM data (psi^4 delta_ij, K_ij = 0)"""
+        L = lib()
+        _sig(L, "bamhip_punctures_bl", C.c_int, [C.c_void_p, C.c_int, DP, DP, IP, IP])
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        p = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1)
+        check(L.bamhip_punctures_bl(self.h, len(m), m.ctypes.data_as(DP), p.ctypes.data_as(DP), _iarr(adm_g),
+                                    _iarr(adm_K)), "bamhip_punctures_bl")
+
+    def adm_to_z4c(self, adm_g, adm_K, u, chipsipower=-4.0, initial_lapse=2, zero_shift=True):
+        """z4 initialisation: lapse (0 unchanged, 1 one, 2 precollapsed), shift, z4_init, G boundary"""
+        L = lib()
+        _sig(L, "bamhip_adm_to_z4c", C.c_int, [C.c_void_p, IP, IP, IP, C.c_double, C.c_int, C.c_int])
+        check(L.bamhip_adm_to_z4c(self.h, _iarr(adm_g), _iarr(adm_K), _iarr(u), chipsipower, initial_lapse,
+                                  1 if zero_shift else 0), "bamhip_adm_to_z4c")
+
     def z4c_to_adm(self, u, adm_g, adm_K, chipsipower=-4.0):
         """z4_to_adm_vl: ADM g_ij, K_ij from the Z4c fields"""
         L = lib()

@@ -473,6 +473,22 @@ int bamhip_grhd_sources(bamhip_level *lev, const bamhip_hrsc_params *par, double
 int bamhip_z4c_matter_source(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, double c,
                              const int *ucur, const int *madm);
 
+/* Brill-Lindquist puncture data (SetMWBL, src/projects/punctures/
+ * ConformalFactor.c:84-200, with the conformal factor absorbed into the
+ * metric, adm_undo_conftrans, src/projects/adm/adm.c:96-111): at every point
+ * psi = 1 + sum_i m_i / (2 r_i), adm_g = psi^4 delta_ij, adm_K = 0.
+ * pos[3 n]: puncture positions, n <= 16. */
+int bamhip_punctures_bl(bamhip_level *lev, int n, const double *mass, const double *pos, const int *adm_g,
+                        const int *adm_K);
+/* The z4 project's POST_INITIALDATA (src/projects/z4/z4.c:280-330): lapse
+ * (initial_lapse 0 unchanged, 1 one, 2 precollapsed = 1/sqrt(adm_gxx)),
+ * shift and B^i zeroed when zero_shift, then adm_to_z4 (z4.c:142-171) =
+ * z4_init (z4_init.m:9-80): the Z4c fields from adm_g / adm_K at every point,
+ * G^i at the inner points, and z4_setbound_G's extrapolation
+ * (extrapolate.c:22-47) on boxes whose faces are all PHYBOUND. */
+int bamhip_adm_to_z4c(bamhip_level *lev, const int *adm_g, const int *adm_K, const int *ucur, double chipsipower,
+                      int initial_lapse, int zero_shift);
+
 /* Replaces z4_to_adm_vl(unew, upre, c, ucur) (src/projects/z4/z4.c:93-124),
  * the "geometry_adm" step of evolve_rhs: at every point psi =
  * chi^(1/chipsipower), adm_g = psi^4 g~_ij, adm_K = psi^4 A~_ij + Khat g_ij / 3.

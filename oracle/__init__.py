@@ -79,7 +79,8 @@ def lib():
                                           C.POINTER(DP), C.c_double, C.POINTER(DP)]
         L.ob_periodic_fill.argtypes = [C.POINTER(Box), C.c_int * 3, C.c_int, DP]
         L.ob_z4c_algcon.argtypes = [C.POINTER(Box), C.c_int, C.c_int, C.POINTER(DP), C.POINTER(DP)]
-        L.ob_adm_to_z4c.argtypes = [C.POINTER(Box), C.c_double, C.POINTER(DP), C.POINTER(DP), C.POINTER(DP)]
+        L.ob_adm_to_z4c.argtypes = [C.POINTER(Box), C.c_double, C.POINTER(DP), C.POINTER(DP), C.POINTER(DP), C.c_int]
+        L.ob_punctures_bl.argtypes = [C.POINTER(Box), C.c_int, DP, DP, C.POINTER(DP), C.POINTER(DP), DP]
         L.ob_z4c_to_adm.argtypes = [C.POINTER(Box), C.c_double, C.POINTER(DP), C.POINTER(DP), C.POINTER(DP)]
         L.ob_z4c_rk4_step.restype = C.c_int
         L.ob_z4c_rk4_step.argtypes = [C.POINTER(Box), C.POINTER(Z4cParams), C.c_double, C.c_int,
@@ -234,11 +235,23 @@ def z4c_algcon(box, u, subtractA=1, normalizedetg=1):
     lib().ob_z4c_algcon(C.byref(box), subtractA, normalizedetg, _ptrs(u[0:6]), _ptrs(u[7:13]))
 
 
-def adm_to_z4c(box, gadm, kadm, chipsipower=-4.0):
+def adm_to_z4c(box, gadm, kadm, chipsipower=-4.0, phybound=False):
+    """z4_init; phybound: extrapolate G on the outermost layer (z4_setbound_G)"""
     u = np.zeros((NV,) + gadm.shape[1:])
     lib().ob_adm_to_z4c(C.byref(box), chipsipower, _ptrs(np.ascontiguousarray(gadm)),
-                        _ptrs(np.ascontiguousarray(kadm)), _ptrs(u))
+                        _ptrs(np.ascontiguousarray(kadm)), _ptrs(u), 1 if phybound else 0)
     return u
+
+
+def punctures_bl(box, masses, positions):
+    """Brill-Lindquist ADM data: (g_ij (6, o, n, m), K_ij, psi)"""
+    shape = (box.o, box.n, box.m)
+    g, k, psi = np.zeros((6,) + shape), np.zeros((6,) + shape), np.zeros(shape)
+    m = np.ascontiguousarray(masses, dtype=np.float64)
+    pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1)
+    lib().ob_punctures_bl(C.byref(box), len(m), m.ctypes.data_as(DP), pos.ctypes.data_as(DP), _ptrs(g), _ptrs(k),
+                          psi.ctypes.data_as(DP))
+    return g, k, psi
 
 
 def z4c_rk4_step(box, params, dt, u, bc=1, pgh=2, algcon=True, work=None):

@@ -81,9 +81,11 @@ void ob_z4c_algcon(const ob_box *b, int subtractA, int normalizedetg,
                    double *const *g6, double *const *A6);
 
 /* ADM -> Z4c (z4_init.m) and Z4c -> ADM (z4.c:93-124) */
-void ob_adm_to_z4c(const ob_box *b, double chipsipower,
-                   const double *const *gadm, const double *const *kadm,
-                   double *const *u);
+void ob_adm_to_z4c(const ob_box *b, double chipsipower, const double *const *gadm, const double *const *kadm,
+                   double *const *u, int phybound);
+/* Brill-Lindquist ADM data (SetMWBL, adm_undo_conftrans) */
+void ob_punctures_bl(const ob_box *b, int n, const double *mass, const double *pos, double *const *gadm,
+                     double *const *kadm, double *psi_out);
 void ob_z4c_to_adm(const ob_box *b, double chipsipower, const double *const *u,
                    double *const *gadm, double *const *kadm);
 

@@ -186,13 +186,59 @@ void ob_z4c_algcon(const ob_box *b, int subtractA, int normalizedetg,
  * At = detg^-1/3 (K - g trK/3), chi = psi^chipsipower with psi = detg^1/12,
  * Khat = trK, Theta = 0, G^i = -d_j gtinv^ji with 2nd-order stencils at inner
  * points (zero on the outermost layer, which z4_setbound_G extrapolates). */
-void ob_adm_to_z4c(const ob_box *b, double chipsipower,
-                   const double *const *gadm, const double *const *kadm,
-                   double *const *u)
+/* set_boundary_extrapolate (src/utility/boundary/extrapolate.c:10-47) on the
+ * outermost point layer of a box whose faces are all PHYBOUND: three passes
+ * in point order, in place, each point from linear extrapolation along the
+ * axes on which a neighbour is missing (set_nbinds_check_7 / check2_6), the
+ * mean of those, or of the three centred means when none is missing */
+static void extrapolate_outer_layer(const ob_box *b, double *v)
+{
+  const long di = 1, dj = b->m, dk = (long)b->m * b->n;
+  for (int rep = 0; rep < 3; rep++)
+    for (int k = 0; k < b->o; k++)
+      for (int j = 0; j < b->n; j++)
+        for (int i = 0; i < b->m; i++) {
+          if (i > 0 && j > 0 && k > 0 && i < b->m - 1 && j < b->n - 1 && k < b->o - 1) continue;
+          const long c = i + dj * j + dk * k;
+          const long nb[3][4] = {
+              {i > 0 ? c - di : c, i > 1 ? c - 2 * di : c, i + 1 < b->m ? c + di : c, i + 2 < b->m ? c + 2 * di : c},
+              {j > 0 ? c - dj : c, j > 1 ? c - 2 * dj : c, j + 1 < b->n ? c + dj : c, j + 2 < b->n ? c + 2 * dj : c},
+              {k > 0 ? c - dk : c, k > 1 ? c - 2 * dk : c, k + 1 < b->o ? c + dk : c, k + 2 < b->o ? c + 2 * dk : c}};
+          double val[3];
+          int use[3];
+          for (int d = 0; d < 3; d++) {
+            const long m1 = nb[d][0], M2 = nb[d][1], p1 = nb[d][2], P2 = nb[d][3];
+            if (p1 == c) { /* extrapolate from the left */
+              val[d] = 2.0 * v[m1] - v[M2];
+              use[d] = 1;
+            } else if (m1 == c) { /* from the right */
+              val[d] = 2.0 * v[p1] - v[P2];
+              use[d] = 1;
+            } else {
+              val[d] = 0.5 * (v[p1] + v[m1]);
+              use[d] = 0;
+            }
+          }
+          if (use[0] + use[1] + use[2] > 0)
+            v[c] = (use[0] * val[0] + use[1] * val[1] + use[2] * val[2]) / (use[0] + use[1] + use[2]);
+          else
+            v[c] = (val[0] + val[1] + val[2]) / 3.0;
+        }
+}
+
+/* ADM -> Z4c: z4_init (src/projects/z4/z4_init.m:9-80) as adm_to_z4 runs it
+ * (z4.c:142-171): at every point detg, psi^-4 = detg^(-1/3), psi =
+ * detg^(1/12), gt = psi^-4 g, K = gt^ij psi^-4 K_ij, At = psi^-4 (K_ij -
+ * g_ij K / 3), chi = psi^chipsipower, Theta = 0; at the inner points (flag
+ * layers excluded) G^a = -d_b gt^{ba} with the generated 2nd-order form
+ * oo2dx (-u[-1] + u[1]); then z4_setbound_G (z4.c:33-39): the outermost layer
+ * extrapolated when phybound is set (a box with PHYBOUND faces). */
+void ob_adm_to_z4c(const ob_box *b, double chipsipower, const double *const *gadm, const double *const *kadm,
+                   double *const *u, int phybound)
 {
   const long np = (long)b->m * b->n * b->o;
   const long st[3] = {1, b->m, (long)b->m * b->n};
-  const double h[3] = {b->dx, b->dy, b->dz};
+  const double oo2h[3] = {1 / (2 * b->dx), 1 / (2 * b->dy), 1 / (2 * b->dz)};
   double *gtinv = malloc(sizeof(double) * 6 * np);
   for (long q = 0; q < np; q++) {
     double g[3][3], K[3][3], gt[3][3], gti[3][3];
@@ -219,22 +265,53 @@ void ob_adm_to_z4c(const ob_box *b, double chipsipower,
     u[OV_CHI][q] = pow(psi, chipsipower);
     u[OV_THETA][q] = 0;
   }
-  for (int k = 0; k < b->o; k++)
-    for (int j = 0; j < b->n; j++)
-      for (int i = 0; i < b->m; i++) {
+  for (int k = b->skip[4]; k < b->o - b->skip[5]; k++)
+    for (int j = b->skip[2]; j < b->n - b->skip[3]; j++)
+      for (int i = b->skip[0]; i < b->m - b->skip[1]; i++) {
         const long q = i + st[1] * j + st[2] * k;
-        const int inner = i > 0 && j > 0 && k > 0 && i < b->m - 1 && j < b->n - 1 && k < b->o - 1;
         for (int a = 0; a < 3; a++) {
           double s = 0;
-          if (inner)
-            for (int c = 0; c < 3; c++) {
-              const double *gi = gtinv + SY[c][a] * np;
-              s -= (gi[q + st[c]] - gi[q - st[c]]) / (2 * h[c]);
-            }
+          for (int c = 0; c < 3; c++) {
+            const double *gi = gtinv + SY[c][a] * np;
+            s -= oo2h[c] * (-gi[q - st[c]] + gi[q + st[c]]);
+          }
           u[OV_GX + a][q] = s;
         }
       }
+  if (phybound)
+    for (int a = 0; a < 3; a++) extrapolate_outer_layer(b, u[OV_GX + a]);
   free(gtinv);
+}
+
+/* Brill-Lindquist data: SetMWBL with mwblpsiofxyz (src/projects/punctures/
+ * ConformalFactor.c:84-200; psi = sum_i (m_i (1/r_i)) 0.5 + 1, r = EPSZERO
+ * where r equals 0 within dequaleps), K_ij = 0, and the conformal factor
+ * absorbed into the metric (adm_undo_conftrans, adm.c:96-111): g_ij =
+ * psi^4 delta_ij.  pos[3 n]: puncture positions; psi may be NULL. */
+void ob_punctures_bl(const ob_box *b, int n, const double *mass, const double *pos, double *const *gadm,
+                     double *const *kadm, double *psi_out)
+{
+  for (int k = 0; k < b->o; k++)
+    for (int j = 0; j < b->n; j++)
+      for (int i = 0; i < b->m; i++) {
+        const long q = i + (long)b->m * j + (long)b->m * b->n * k;
+        const double x = b->x0 + i * b->dx, y = b->y0 + j * b->dy, z = b->z0 + k * b->dz;
+        double p = 0;
+        for (int c = 0; c < n; c++) {
+          const double xi = x - pos[3 * c], yi = y - pos[3 * c + 1], zi = z - pos[3 * c + 2];
+          double r = sqrt(xi * xi + yi * yi + zi * zi);
+          if (!(r < 0.0 - 1e-10 || 0.0 < r - 1e-10)) r = 1e-10 / 1000.0;
+          const double ri = 1 / r;
+          p += mass[c] * ri * (0.5);
+        }
+        p += 1.0;
+        const double psi4 = pow(p, 4.0);
+        for (int a = 0; a < 6; a++) {
+          gadm[a][q] = (a == 0 || a == 3 || a == 5 ? 1.0 : 0.0) * psi4;
+          kadm[a][q] = 0.0 * pow(p, -2.0);
+        }
+        if (psi_out) psi_out[q] = p;
+      }
 }
 
 /* Z4c -> ADM (z4.c:93-124): psi = chi^(1/chipsipower), g = psi^4 gt,
