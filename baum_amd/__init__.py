@@ -554,6 +554,44 @@ class C2PParams(C.Structure):
                 ("countmax", C.c_int)]
 
 
+class CheckpointStruct(C.Structure):
+    _fields_ = [("lmin", C.c_int), ("lmax", C.c_int), ("levels", C.POINTER(C.c_void_p)), ("iteration", IP),
+                ("time", DP), ("nparams", C.c_int), ("param_names", C.POINTER(C.c_char_p)),
+                ("param_values", C.POINTER(C.c_char_p)), ("nvars", C.c_int), ("vars", IP),
+                ("var_names", C.POINTER(C.c_char_p))]
+
+
+def _checkpoint_struct(levels, iterations, times, vars, names, params, lmin):
+    n = len(levels)
+    keep = dict(lv=(C.c_void_p * n)(*[lv.h for lv in levels]), it=(C.c_int * n)(*iterations),
+                tm=(C.c_double * n)(*times), pn=(C.c_char_p * max(len(params), 1))(*[k.encode() for k in params]),
+                pv=(C.c_char_p * max(len(params), 1))(*[str(v).encode() for v in params.values()]),
+                vr=(C.c_int * max(len(vars), 1))(*vars), vn=(C.c_char_p * max(len(names), 1))(*[x.encode() for x in names]))
+    S = CheckpointStruct(lmin, lmin + n - 1, C.cast(keep["lv"], C.POINTER(C.c_void_p)), C.cast(keep["it"], IP),
+                         C.cast(keep["tm"], DP), len(params), keep["pn"], keep["pv"], len(vars),
+                         C.cast(keep["vr"], IP), keep["vn"])
+    return S, keep
+
+
+def checkpoint_write(path, levels, iterations, times, vars, names, params=None, lmin=0):
+    """checkpoint_write_local for device levels (the reference's file layout)"""
+    L = lib()
+    _sig(L, "bamhip_checkpoint_write", C.c_int, [C.c_char_p, C.POINTER(CheckpointStruct)])
+    S, keep = _checkpoint_struct(levels, iterations, times, vars, names, params or {}, lmin)
+    check(L.bamhip_checkpoint_write(str(path).encode(), C.byref(S)), "bamhip_checkpoint_write")
+
+
+def checkpoint_read(path, levels, vars, names, lmin=0):
+    """checkpoint_read into device levels; returns (iterations, times) per level"""
+    L = lib()
+    _sig(L, "bamhip_checkpoint_read", C.c_int, [C.c_char_p, C.POINTER(CheckpointStruct), IP, DP])
+    n = len(levels)
+    S, keep = _checkpoint_struct(levels, [0] * n, [0.0] * n, vars, names, {}, lmin)
+    it, tm = (C.c_int * n)(), (C.c_double * n)()
+    check(L.bamhip_checkpoint_read(str(path).encode(), C.byref(S), it, tm), "bamhip_checkpoint_read")
+    return list(it), list(tm)
+
+
 class HrscParams(C.Structure):
     _fields_ = [("eos", C.c_int), ("gamma", C.c_double), ("K", C.c_double), ("vmax", C.c_double),
                 ("wlormax", C.c_double), ("riemann", C.c_int), ("rec", C.c_int)]

@@ -557,6 +557,31 @@ int bamhip_output_0d(bamhip_level *lev, int nv, const int *vars, const char *con
 int bamhip_output_1d(bamhip_level *lev, int nv, const int *vars, const char *const *names, int line, double x0,
                      double y0, double z0, double time, const char *outdir);
 
+/* ----------------------------------------------------------- checkpoints */
+/* checkpoint_write_local / checkpoint_read (src/utility/checkpoint/
+ * checkpoint_write.c:91-205, checkpoint_read.c:96-275) for device levels:
+ * the reference's file layout (text header with iteration and time per
+ * level, the parameter database as name = value lines, then per level and
+ * enabled variable a "$variable = name : level = l , level->nnodes = n" line
+ * and n binary doubles).  levels[l - lmin], iteration[] / time[] per level,
+ * vars / var_names the variables to write or to restore by name.  Reading
+ * fills iteration / time (may be NULL) and the listed, enabled variables; a
+ * point count other than the level's is an error, the parameter database
+ * and unlisted variables are skipped. */
+typedef struct {
+  int lmin, lmax;
+  bamhip_level *const *levels;
+  const int *iteration;
+  const double *time;
+  int nparams;
+  const char *const *param_names, *const *param_values;
+  int nvars;
+  const int *vars;
+  const char *const *var_names;
+} bamhip_checkpoint;
+int bamhip_checkpoint_write(const char *path, const bamhip_checkpoint *C);
+int bamhip_checkpoint_read(const char *path, const bamhip_checkpoint *C, int *iteration, double *time);
+
 /* --------------------------------------------------------------- graphs */
 /* Capture the device work issued on the library stream between begin and
  * end into an executable graph (hipGraph), replayed by launch.  Timers are
