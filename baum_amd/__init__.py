@@ -235,6 +235,28 @@ class Level:
             self.offsets.append(off)
             off += b.m * b.n * b.o
 
+    @classmethod
+    def adopt(cls, handle, nvariables):
+        """wrap a level the library created (a level bamhip_move_box made)"""
+        L = lib()
+        _sig(L, "bamhip_level_info", C.c_int, [C.c_void_p, IP, IP])
+        _sig(L, "bamhip_level_box", C.c_int, [C.c_void_p, C.c_int, C.POINTER(Box), C.POINTER(C.c_long)])
+        self = cls.__new__(cls)
+        self.h = C.c_void_p(handle)
+        lv, nb = C.c_int(), C.c_int()
+        check(L.bamhip_level_info(self.h, C.byref(lv), C.byref(nb)), "bamhip_level_info")
+        self.boxes, self.offsets = [], []
+        for b in range(nb.value):
+            bx, off = Box(), C.c_long()
+            check(L.bamhip_level_box(self.h, b, C.byref(bx), C.byref(off)), "bamhip_level_box")
+            self.boxes.append(bx)
+            self.offsets.append(off.value)
+        n = C.c_long()
+        check(L.bamhip_level_npoints(self.h, C.byref(n)), "bamhip_level_npoints")
+        self.npoints = n.value
+        self.nvariables = nvariables
+        return self
+
     def free(self):
         if self.h:
             lib().bamhip_level_free(self.h)
@@ -471,7 +493,32 @@ class FmrHierarchyStruct(C.Structure):
                 ("dt0", C.c_double), ("iteration", C.POINTER(C.c_int)),
                 ("u", IP), ("up", IP), ("upp", IP), ("r", IP), ("v", IP),
                 ("order_RP", C.c_int), ("timeinterpolation", C.c_int), ("pgh", C.c_int),
-                ("npunc", C.c_int), ("punc", C.c_void_p), ("punc_flags", C.c_int)]
+                ("npunc", C.c_int), ("punc", C.c_void_p), ("punc_flags", C.c_int),
+                ("move_nxyz", C.c_int), ("move_lcube", C.c_int),
+                ("nbuffer", C.c_int), ("fp", C.c_int), ("fr", C.c_int)]
+
+
+def find_nested_bboxes(points, h, nxyz):
+    """bamhip_find_nested_bboxes (host only): the coarse bounding boxes
+    (nboxes, 6) and index boxes around the points (np, 3) on spacing h"""
+    L = lib()
+    _sig(L, "bamhip_find_nested_bboxes", C.c_int, [C.c_int, DP, DP, C.c_int, IP, DP, IP])
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    hh = np.ascontiguousarray(np.broadcast_to(np.asarray(h, dtype=np.float64), (3,)))
+    nb = C.c_int()
+    bb = (C.c_double * 60)()
+    ib = (C.c_int * 60)()
+    check(L.bamhip_find_nested_bboxes(p.shape[0], p.ctypes.data_as(DP), hh.ctypes.data_as(DP), int(nxyz),
+                                      C.byref(nb), bb, ib), "bamhip_find_nested_bboxes")
+    n = nb.value
+    return np.array(bb[:6 * n]).reshape(n, 6), np.array(ib[:6 * n]).reshape(n, 6)
+
+
+def copy_overlap(lold, lnew, vars_):
+    """bamhip_level_copy_overlap: the points the levels share, old -> new"""
+    L = lib()
+    _sig(L, "bamhip_level_copy_overlap", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, IP])
+    check(L.bamhip_level_copy_overlap(lold.h, lnew.h, len(vars_), _iarr(vars_)), "bamhip_level_copy_overlap")
 
 
 class Hierarchy:
@@ -480,7 +527,7 @@ class Hierarchy:
     params: one Z4cParams per level."""
 
     def __init__(self, levels, params, dt0, u, up, upp, r, v, order_RP=4, timeinterpolation=3, pgh=0,
-                 punctures=None, pin_z=False):
+                 punctures=None, pin_z=False, move_nxyz=0, move_lcube=0, rp_flags=(3, -1, -1)):
         L = lib()
         hp = C.POINTER(FmrHierarchyStruct)
         _sig(L, "bamhip_advance_levelandsublevels", C.c_int, [hp, C.c_int])
@@ -508,14 +555,37 @@ class Hierarchy:
         self.punctures = punctures
         if punctures is not None:
             s.npunc, s.punc, s.punc_flags = punctures.shape[0], punctures.ptr, 1 if pin_z else 0
+        s.move_nxyz, s.move_lcube = move_nxyz, move_lcube
+        s.nbuffer, s.fp, s.fr = rp_flags
         self.s = s
+
+    def _adopt_moved(self):
+        """wrap the levels bamhip_move_box replaced (the old ones are freed)"""
+        for l, lv in enumerate(self.levels):
+            if self._lv[l] != lv.h.value:
+                lv.h = None
+                self.levels[l] = Level.adopt(self._lv[l], lv.nvariables)
+
+    def move_box(self, l):
+        """move_box(g, l): the levels from l down follow the punctures;
+        returns whether a level moved"""
+        L = lib()
+        _sig(L, "bamhip_move_box", C.c_int, [C.POINTER(FmrHierarchyStruct), C.c_int, IP])
+        moved = C.c_int()
+        rc = L.bamhip_move_box(C.byref(self.s), l, C.byref(moved))
+        self._adopt_moved()
+        check(rc, "bamhip_move_box")
+        return bool(moved.value)
 
     def set_check(self, mode):
         check(lib().bamhip_fmr_set_check(C.byref(self.s), mode), "bamhip_fmr_set_check")
 
     def advance(self, l=0):
         """advance_levelandsublevels(l): one step of level l and its sublevels"""
-        check(lib().bamhip_advance_levelandsublevels(C.byref(self.s), l), "bamhip_advance_levelandsublevels")
+        rc = lib().bamhip_advance_levelandsublevels(C.byref(self.s), l)
+        if self.s.move_nxyz > 0:
+            self._adopt_moved()
+        check(rc, "bamhip_advance_levelandsublevels")
 
     def nonfinite(self, reset=True):
         f = C.c_int()

@@ -10,7 +10,8 @@
  * time (parentaligned, inject.c:10-19).  Subcycling on every level (amr =
  * bo, lbo = 0).  After the finest level's step and its restriction, the
  * puncture tracker of analyze_level (main.c:419-423; track_puncture,
- * moving_punctures.c:111-196).
+ * moving_punctures.c:111-196), and move_box for the levels below (main.c:
+ * 425-436, bamhip_move_box) when the boxes follow the punctures.
  *
  * The whole recursion issues device work only: the levels run with the
  * non-finite check deferred, so a coarse step can be captured as a graph.
@@ -58,6 +59,8 @@ int bamhip_advance_levelandsublevels(const bamhip_fmr_hierarchy *H, int l)
                                    H->punc)))
       return e;
   }
+  /* move boxes (main.c:425-436) */
+  if (H->move_nxyz > 0 && (e = bamhip_move_box(H, l + 1, NULL))) return e;
   return BAMHIP_OK;
 }
 

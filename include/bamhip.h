@@ -356,10 +356,14 @@ int bamhip_bo_interp_end(bamhip_level *lev, int timeinterpolation, int iteration
  * puncture coordinates, tracked on level lmax after each of its steps
  * (track_puncture in analyze_level, main.c:415-423, with lmaxpunc = lmax)
  * with the shift u[19..21] / up[19..21], interpolation order order_RP and
- * punc_flags as bamhip_puncture_track's flags. */
+ * punc_flags as bamhip_puncture_track's flags.  move_nxyz > 0 (amr = move,
+ * amr_move_nxyz): the fine levels follow the punctures (bamhip_move_box),
+ * which replaces entries of levels[]; move_lcube is amr_move_lcube (levels
+ * up to it stay centred on the origin); nbuffer, fp, fr are
+ * bamhip_set_rp_flags' amr_nbuffer and flag variables for the new levels. */
 typedef struct {
   int lmax;
-  bamhip_level *const *levels;
+  bamhip_level **levels;
   const bamhip_z4c_params *params;
   double dt0;
   int *iteration;
@@ -370,18 +374,55 @@ typedef struct {
   int npunc;               /* 0: no tracking */
   double *punc;            /* device, 3*npunc */
   int punc_flags;
+  int move_nxyz;           /* 0: boxes stay where they are */
+  int move_lcube;
+  int nbuffer, fp, fr;
 } bamhip_fmr_hierarchy;
 
 /* Replaces advance_levelandsublevels(g, l) (src/main/main/main.c:397-453)
  * for the Z4c list: one step of level l (time interpolation of u_pp around
  * an RK4 step, evolve.c:298-380, then algcon), two steps of every finer
  * level, then restrict_prolong_evolve with the parent (inject.c:116-132),
- * then on level lmax the puncture tracker.  Device work only: the RHS non-finite check is deferred (levels must be in
- * check mode 0, see bamhip_fmr_set_check), so the call can be captured. */
+ * then on level lmax the puncture tracker, then move_box(l + 1)
+ * (main.c:425-436) when move_nxyz > 0.  Without moving boxes it issues
+ * device work only: the RHS non-finite check is deferred (levels must be in
+ * check mode 0, see bamhip_fmr_set_check), so the call can be captured; a
+ * box move synchronises and allocates, so such calls cannot. */
 int bamhip_advance_levelandsublevels(const bamhip_fmr_hierarchy *H, int l);
 int bamhip_fmr_set_check(const bamhip_fmr_hierarchy *H, int mode);
 /* OR of the levels' non-finite flags (synchronises); reset clears them */
 int bamhip_fmr_nonfinite(const bamhip_fmr_hierarchy *H, int reset, int *flag);
+
+/* ---------------------------------------------------------- moving boxes */
+/* find_nested_bboxes (src/main/amr/move.c:121-465) without symmetries or
+ * forced motion: around each of the np (<= 10) points pos[3*j..] snapped
+ * to the coarse spacing h[3] by truncation, a box of n = nxyz/2 rounded up
+ * to even coarse points per direction; overlapping boxes are joined in the
+ * reference's pairwise order.  Out: *nboxes, bbox[6*b] (coarse coordinates
+ * of the first and last point per direction), ibbox[6*b] (0, cells, ...).
+ * Host only.  nxyz must be even and positive. */
+int bamhip_find_nested_bboxes(int np, const double *pos, const double *h, int nxyz, int *nboxes, double *bbox,
+                              int *ibbox);
+/* move_overlap_nonparallel (move.c:589-676) with copyall: for every pair
+ * (old box, new box) in that order, copy the n variables at the points the
+ * two boxes share (index ranges of box_overlap_range, move.c:487-533);
+ * variables not enabled on both levels are skipped.  Same spacing and
+ * aligned points required. */
+int bamhip_level_copy_overlap(bamhip_level *lold, bamhip_level *lnew, int n, const int *vars);
+/* move_box(g, l) (move.c:808-988) for the hierarchy: returns at once
+ * unless move_nxyz > 0, 1 <= l <= lmax and level l-1 is the coarsest of its
+ * time (time[l-1] < time[l-2]); then from l to lmax the nested boxes of
+ * each level are found on its parent around the punctures (the origin up
+ * to move_lcube); a level whose boxes differ is replaced by a new one
+ * (make_level_bboxes, box.c:465-560: 2*(cells+1) points per direction,
+ * first point half a fine spacing inside the coarse box, REFBOUND faces)
+ * with the old level's enabled variables, filled by order_RP prolongation
+ * everywhere from the parent (prolong_varlist with nbuffer -1) of u, up and
+ * upp, then by the copy of the overlap with the old level (the reference's
+ * global and local methods give these values); a level kept in place is
+ * linked again to a replaced parent.  The box comparison is with the
+ * level's present boxes.  *moved (may be NULL) = 1 if a level moved. */
+int bamhip_move_box(const bamhip_fmr_hierarchy *H, int l, int *moved);
 
 /* RK4 stages without algcon and without the non-finite check, and algcon on
  * the 12 fields of a Z4c list (building blocks of the drivers above) */
