@@ -365,6 +365,20 @@ class Level:
         check(L.bamhip_punctures_bl(self.h, len(m), m.ctypes.data_as(DP), p.ctypes.data_as(DP), _iarr(adm_g),
                                     _iarr(adm_K)), "bamhip_punctures_bl")
 
+    def tov_set(self, star, adm_g, adm_K, alpha, prim, center=(0.0, 0.0, 0.0), riso=None, rsch=None, phi=None,
+                h=None):
+        """set_TOV: the star of tov_h(...) (or explicit tables) on the level;
+        prim = rho, epsl, vx, vy, vz, p"""
+        L = lib()
+        _sig(L, "bamhip_tov_set", C.c_int, [C.c_void_p, C.c_int, DP, DP, DP, DP, C.c_double, C.c_double,
+                                             C.c_double, DP, IP, IP, C.c_int, IP])
+        t = [np.ascontiguousarray(star[k] if a is None else a, dtype=np.float64)
+             for k, a in (("riso", riso), ("r", rsch), ("phi", phi), ("h", h))]
+        c = np.ascontiguousarray(center, dtype=np.float64)
+        check(L.bamhip_tov_set(self.h, t[0].size, *[a.ctypes.data_as(DP) for a in t], star["M"], star["K"],
+                               star["Gamma"], c.ctypes.data_as(DP), _iarr(adm_g), _iarr(adm_K), int(alpha),
+                               _iarr(prim)), "bamhip_tov_set")
+
     def adm_to_z4c(self, adm_g, adm_K, u, chipsipower=-4.0, initial_lapse=2, zero_shift=True):
         """z4 initialisation: lapse (0 unchanged, 1 one, 2 precollapsed), shift, z4_init, G boundary"""
         L = lib()
@@ -512,6 +526,20 @@ def find_nested_bboxes(points, h, nxyz):
                                       C.byref(nb), bb, ib), "bamhip_find_nested_bboxes")
     n = nb.value
     return np.array(bb[:6 * n]).reshape(n, 6), np.array(ib[:6 * n]).reshape(n, 6)
+
+
+def tov_h(hc, K=100.0, Gamma=2.0, npts=100000):
+    """bamhip_tov_h (host only): the TOV star of the polytrope integrated in
+    the log enthalpy; returns the tables and M, R, Riso, Mb"""
+    L = lib()
+    _sig(L, "bamhip_tov_h", C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int] + [DP] * 8)
+    t = {k: np.empty(npts) for k in ("r", "m", "h", "rho", "pre", "phi", "riso")}
+    out = np.empty(4)
+    check(L.bamhip_tov_h(hc, K, Gamma, npts, *[t[k].ctypes.data_as(DP) for k in ("r", "m", "h", "rho", "pre", "phi",
+                                                                              "riso")],
+                         out.ctypes.data_as(DP)), "bamhip_tov_h")
+    t.update(M=out[0], R=out[1], Riso=out[2], Mb=out[3], K=K, Gamma=Gamma)
+    return t
 
 
 def copy_overlap(lold, lnew, vars_):

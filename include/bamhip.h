@@ -598,6 +598,26 @@ int bamhip_output_0d(bamhip_level *lev, int nv, const int *vars, const char *con
 int bamhip_output_1d(bamhip_level *lev, int nv, const int *vars, const char *const *names, int line, double x0,
                      double y0, double z0, double time, const char *outdir);
 
+/* tov_h (src/projects/BBIDdataReader/tov.c:104-236, BBID_tov_integrate = h)
+ * for the polytrope p = K rho^Gamma: npts points from the centre enthalpy
+ * hc to the surface, per point the Schwarzschild radius r, the mass m, the
+ * log enthalpy h, rho, pre, the potential phi and the isotropic radius
+ * riso.  out[4] (may be NULL): M, R (Schwarzschild), R (isotropic) and the
+ * baryonic mass (computeMb, tov.c:10-24).  Host only. */
+int bamhip_tov_h(double hc, double K, double Gamma, int npts, double *r, double *m, double *h, double *rho,
+                 double *pre, double *phi, double *riso, double *out);
+/* set_TOV (BBIDdataReader.c:483-690) without boost or perturbation: the
+ * star of the host table (n points of bamhip_tov_h's riso, r, phi, h; mass
+ * M) centred at center[3] on every point of the level: inside the surface
+ * riso[n-1] the table read by interp_lag4 (:280-318) at the isotropic
+ * radius, alpha = exp(phi), g_ij = (r/riso)^2 delta_ij and the matter of the
+ * enthalpy; outside, the exterior Schwarzschild solution in isotropic
+ * coordinates and no matter.  adm_g[6], adm_K[6] (set to 0), alpha, prim[6]
+ * = rho, epsl, vx, vy, vz (0), p. */
+int bamhip_tov_set(bamhip_level *lev, int n, const double *riso, const double *rsch, const double *phi,
+                   const double *h, double M, double K, double Gamma, const double *center, const int *adm_g,
+                   const int *adm_K, int alpha, const int *prim);
+
 /* ----------------------------------------------------------- checkpoints */
 /* checkpoint_write_local / checkpoint_read (src/utility/checkpoint/
  * checkpoint_write.c:91-205, checkpoint_read.c:96-275) for device levels:
