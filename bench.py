@@ -373,42 +373,6 @@ def run_c2p(args):
 
 
 # ------------------------------------------------------------------ C5: coupled Z4c + GRHD
-def tov_isotropic(rhoc, K=100.0, G=2.0, dr=2e-3):
-    """TOV star of the polytrope p = K rho^G: RK4 in the Schwarzschild radius
-    for m, p, nu and the isotropic radius, rescaled to the exterior
-    Schwarzschild solution in isotropic coordinates at the surface"""
-    def rhs(r, y):
-        m, P = y[0], y[1]
-        rho = (max(P, 0.0) / K) ** (1.0 / G)
-        e = rho + P / (G - 1.0)
-        den = r * (r - 2 * m)
-        return np.array([4 * np.pi * r * r * e, -(e + P) * (m + 4 * np.pi * r ** 3 * P) / den,
-                         (m + 4 * np.pi * r ** 3 * P) / den, 1.0 / (r * np.sqrt(1 - 2 * m / r))])
-    Pc = K * rhoc ** G
-    r = dr
-    y = np.array([4 / 3 * np.pi * r ** 3 * (rhoc + Pc / (G - 1)), Pc, 0.0, np.log(r)])
-    rs, ys = [r], [y.copy()]
-    while True:
-        k1 = rhs(r, y)
-        k2 = rhs(r + dr / 2, y + dr / 2 * k1)
-        k3 = rhs(r + dr / 2, y + dr / 2 * k2)
-        k4 = rhs(r + dr, y + dr * k3)
-        yn = y + dr / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
-        if yn[1] <= 0:
-            break
-        r, y = r + dr, yn
-        rs.append(r)
-        ys.append(y.copy())
-    rs, ys = np.array(rs), np.array(ys)
-    R, M = rs[-1], ys[-1, 0]
-    rbs = 0.5 * (R - M + np.sqrt(R * (R - 2 * M)))
-    rb = np.exp(ys[:, 3] - ys[-1, 3]) * rbs
-    return dict(M=M, R=R, rbs=rbs, rb=np.concatenate([[0.0], rb]),
-                alpha=np.concatenate([[0.0], np.exp(ys[:, 2] - ys[-1, 2]) * np.sqrt(1 - 2 * M / R)]),
-                psi4=np.concatenate([[0.0], (rs / rb) ** 2]),
-                rho=np.concatenate([[rhoc], (np.maximum(ys[:, 1], 0) / K) ** (1 / G)]))
-
-
 def run_c5(args, net):
     """the coupled Z4c + GRHD system on a 128^3 box holding a TOV star
     (Gamma 2, K 100, rho_c 1.28e-3, 1e-4 density noise): one right-hand side
@@ -420,30 +384,25 @@ def run_c5(args, net):
     ns = n + 2
     x = -L - 0.5 * h + h * np.arange(ns)
     K, G, rhoc = 100.0, 2.0, 1.28e-3
-    star = tov_isotropic(rhoc, K, G)
-    star["alpha"][0], star["psi4"][0] = star["alpha"][1], star["psi4"][1]
-    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
-    rb = np.sqrt(X * X + Y * Y + Z * Z).reshape(-1)
-    del X, Y, Z
-    inside = rb < star["rbs"]
-    M = star["M"]
-    psi4 = np.where(inside, np.interp(rb, star["rb"], star["psi4"]), (1 + M / (2 * np.maximum(rb, 1e-9))) ** 4)
-    alpha = np.where(inside, np.interp(rb, star["rb"], star["alpha"]),
-                     (1 - M / (2 * np.maximum(rb, 1e-9))) / (1 + M / (2 * np.maximum(rb, 1e-9))))
-    rhoatm = 1e-10 * rhoc
-    rho0 = np.where(inside, np.interp(rb, star["rb"], star["rho"]), rhoatm)
-    rho0 = np.maximum(rho0, rhoatm)
-    rng = np.random.default_rng(20261202)
-    rho = rho0 * (1 + 1e-4 * rng.standard_normal(rho0.shape))
-    eps = K * rho ** (G - 1) / (G - 1)
-    sg = psi4 ** 1.5
-    npts = ns ** 3
+    hc = float(np.log1p(G * K / (G - 1) * rhoc ** (G - 1)))     # log enthalpy of rho_c
+    star = B.tov_h(hc, K, G, 100000)
     UZ, UQ = list(range(25)), list(range(25, 30))
     RZ, RQ = list(range(30, 55)), list(range(55, 60))
     ADMG, ADMK, PRIM, MADM = list(range(60, 66)), list(range(66, 72)), list(range(72, 80)), list(range(80, 91))
     UP, V = list(range(91, 121)), list(range(121, 151))
     lev = B.Level([B.make_box(ns, ns, ns, x[0], x[0], x[0], h)], 151)
     lev.enable(range(151))
+    # the star on the grid (set_TOV): psi^4, alpha and rho read back for the noisy matter below
+    lev.tov_set(star, ADMG, ADMK, 18, PRIM[:6])
+    psi4, alpha, rho_star = lev.get(ADMG[0]), lev.get(18), lev.get(PRIM[0])
+    M = star["M"]
+    rhoatm = 1e-10 * rhoc
+    rho0 = np.maximum(rho_star, rhoatm)
+    rng = np.random.default_rng(20261202)
+    rho = rho0 * (1 + 1e-4 * rng.standard_normal(rho0.shape))
+    eps = K * rho ** (G - 1) / (G - 1)
+    sg = psi4 ** 1.5
+    npts = ns ** 3
     for v in range(151):
         lev.fill(v, 0.0)
 
@@ -461,7 +420,7 @@ def run_c5(args, net):
     status, iters = B.DeviceArray((npts,), np.int32), B.DeviceArray((npts,), np.int32)
     fb = B.DeviceArray.from_host(np.zeros(1, np.int32))
     out = {"grid": f"{n}^3 interior + 1 boundary layer, h = {h}", "tov": {"M": star["M"], "R_schwarzschild": star["R"],
-                                                                         "R_isotropic": star["rbs"]}}
+                                                                         "R_isotropic": star["Riso"]}}
 
     def reset():
         lev.put(UQ[0], cons[0])
