@@ -373,6 +373,48 @@ def run_c2p(args):
 
 
 # ------------------------------------------------------------------ C5: coupled Z4c + GRHD
+def run_move():
+    """move_box on a 5-level hierarchy of 96^3 boxes nested around one
+    puncture (coarse 192^3, h0 = 1): the puncture jumps by one coarse cell
+    and every fine level is rebuilt (allocation, flags, prolongation
+    everywhere of u, up, upp, overlap copy of 75 variables)"""
+    import time
+    import baum_amd as B
+    nvar, nxyz, nlev = 127, 96, 5
+    U, UP, UPP, R, V = (list(range(25 * k, 25 * k + 25)) for k in range(5))
+    levels = [B.Level([B.make_box(192, 192, 192, -95.5, -95.5, -95.5, 1.0, bflag=(B.PHYBOUND,) * 6)], nvar)]
+    for l in range(1, nlev):
+        hc = levels[-1].boxes[0].dx
+        bb, ib = B.find_nested_bboxes([[0.0, 0.0, 0.0]], hc, nxyz)
+        d = hc / 2
+        b = B.make_box(2 * (ib[0][1] + 1), 2 * (ib[0][3] + 1), 2 * (ib[0][5] + 1), bb[0][0] - d / 2, bb[0][2] - d / 2,
+                       bb[0][4] - d / 2, d, bflag=(B.REFBOUND,) * 6)
+        levels.append(B.Level([b], nvar, l=l))
+    for lv in levels:
+        lv.enable(range(nvar))
+        for v in range(nvar):
+            lv.fill(v, 1.0 if v in (6, 18) else 0.0)
+    for l in range(1, nlev):
+        B.set_rp_flags(levels[l - 1], levels[l], [0], 6, 125, 126)
+    punc = B.DeviceArray.from_host(np.zeros((1, 3)))
+    H = B.Hierarchy(levels, [B.z4c_params()] * nlev, 0.25, U, UP, UPP, R, V, punctures=punc, move_nxyz=nxyz,
+                    rp_flags=(6, 125, 126))
+    res = []
+    for shift in (1.0, 2.0):
+        punc.put(np.array([[shift, 0.0, 0.0]]))
+        B.synchronize()
+        t0 = time.perf_counter()
+        moved = H.move_box(1)
+        B.synchronize()
+        res.append({"shift": shift, "moved": moved, "ms": 1e3 * (time.perf_counter() - t0)})
+    pts = sum(lv.npoints for lv in H.levels[1:])
+    for lv in H.levels:
+        lv.free()
+    punc.free() if hasattr(punc, "free") else None
+    return {"workload": "5 levels of 96^3 around one puncture, the puncture moved by one coarse cell of level 0; "
+                        "all 4 fine levels rebuilt", "fine_points": pts, "moves": res}
+
+
 def run_c5(args, net):
     """the coupled Z4c + GRHD system on a 128^3 box holding a TOV star
     (Gamma 2, K 100, rho_c 1.28e-3, 1e-4 density noise): one right-hand side
@@ -590,6 +632,12 @@ def main():
             progress("coupled Z4c + GRHD on the TOV star")
             out["c5"] = run_c5(args, net)
         net.free()
+    if "move" in only:
+        progress("moving boxes")
+        try:
+            out["move"] = run_move()
+        except Exception as e:          # reported, never fatal for the bench line
+            out["move"] = {"error": str(e)}
     if not args.no_cpu:
         progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
