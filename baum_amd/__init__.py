@@ -92,6 +92,7 @@ def lib():
     _sig(L, "bamhip_level_fill", i, [vp, i, d])
     _sig(L, "bamhip_level_set_check", i, [vp, i])
     _sig(L, "bamhip_level_nonfinite", i, [vp, i, IP])
+    _sig(L, "bamhip_level_get_check", i, [vp, IP])
     _sig(L, "bamhip_z4c_rhs", i, [vp, C.POINTER(Z4cParams), IP, IP, d, IP])
     _sig(L, "bamhip_z4c_tune_mode", i, [i])
     _sig(L, "bamhip_z4c_tune_clear", i, [])
@@ -386,6 +387,28 @@ class Level:
         check(L.bamhip_adm_to_z4c(self.h, _iarr(adm_g), _iarr(adm_K), _iarr(u), chipsipower, initial_lapse,
                                   1 if zero_shift else 0), "bamhip_adm_to_z4c")
 
+    def set_check(self, mode):
+        """non-finite check mode: 1 synchronise per call, 0 deferred flag"""
+        check(lib().bamhip_level_set_check(self.h, mode), "bamhip_level_set_check")
+
+    def get_check(self):
+        m = C.c_int()
+        check(lib().bamhip_level_get_check(self.h, C.byref(m)), "bamhip_level_get_check")
+        return m.value
+
+    def nonfinite(self, reset=True):
+        f = C.c_int()
+        check(lib().bamhip_level_nonfinite(self.h, 1 if reset else 0, C.byref(f)), "bamhip_level_nonfinite")
+        return f.value
+
+    def grhd_atm_set_prerhs(self, c2p, upre5, prim8):
+        """grhd_atm_set_prerhs: the cold static atmosphere on the conserved
+        variables upre5 (D, tau, S_i) where prim8's rho is below fatm*rhoatm"""
+        L = lib()
+        _sig(L, "bamhip_grhd_atm_set_prerhs", C.c_int, [C.c_void_p, C.POINTER(C2PParams), IP, IP])
+        check(L.bamhip_grhd_atm_set_prerhs(self.h, C.byref(c2p), _iarr(upre5), _iarr(prim8)),
+              "bamhip_grhd_atm_set_prerhs")
+
     def z4c_to_adm(self, u, adm_g, adm_K, chipsipower=-4.0):
         """z4_to_adm_vl: ADM g_ij, K_ij from the Z4c fields"""
         L = lib()
@@ -393,32 +416,34 @@ class Level:
         check(L.bamhip_z4c_to_adm(self.h, _iarr(u), chipsipower, _iarr(adm_g), _iarr(adm_K)), "bamhip_z4c_to_adm")
 
     @staticmethod
-    def _z4c_grhd(z4, c2p, hrsc, adm_g, adm_K, prim, madm, status, iters, method, mlp, polish_max, fallbacks, pgh):
+    def _z4c_grhd(z4, c2p, hrsc, adm_g, adm_K, prim, madm, status, iters, method, mlp, polish_max, fallbacks, pgh,
+                  atm_prerhs=True):
         keep = [_iarr(x) for x in (adm_g, adm_K, prim, madm)]
         S = Z4cGrhdStruct(C.pointer(z4), C.pointer(c2p), C.pointer(hrsc), method, mlp.h if mlp is not None else None,
                           polish_max, fallbacks.ptr if fallbacks is not None else None, pgh,
-                          *[C.cast(k, IP) for k in keep], status.ptr, iters.ptr)
+                          *[C.cast(k, IP) for k in keep], status.ptr, iters.ptr, 1 if atm_prerhs else 0)
         return S, keep
 
     def z4c_grhd_rhs(self, z4, c2p, hrsc, unew, ucur, adm_g, adm_K, prim, madm, status, iters, upre=None, c=0.0,
-                     method=0, mlp=None, polish_max=8, fallbacks=None, pgh=0):
+                     method=0, mlp=None, polish_max=8, fallbacks=None, pgh=0, atm_prerhs=True):
         """evolve_rhs of the coupled Z4c + GRHD system; lists of 30 indices
-        (25 Z4c fields, then D, tau, S_i); status / iters: DeviceArray int32"""
+        (25 Z4c fields, then D, tau, S_i); status / iters: DeviceArray int32;
+        atm_prerhs: the atmosphere reset of upre after the c2p (when upre is given)"""
         L = lib()
         _sig(L, "bamhip_z4c_grhd_rhs", C.c_int, [C.c_void_p, C.POINTER(Z4cGrhdStruct), IP, IP, C.c_double, IP])
         S, keep = self._z4c_grhd(z4, c2p, hrsc, adm_g, adm_K, prim, madm, status, iters, method, mlp, polish_max,
-                                 fallbacks, pgh)
+                                 fallbacks, pgh, atm_prerhs)
         check(L.bamhip_z4c_grhd_rhs(self.h, C.byref(S), _iarr(unew), _iarr(upre) if upre is not None else None, c,
                                     _iarr(ucur)), "bamhip_z4c_grhd_rhs")
 
     def z4c_grhd_rk4_step(self, z4, c2p, hrsc, dt, uc, up, r, v, adm_g, adm_K, prim, madm, status, iters,
-                          method=0, mlp=None, polish_max=8, fallbacks=None, pgh=0, algcon=True):
+                          method=0, mlp=None, polish_max=8, fallbacks=None, pgh=0, algcon=True, atm_prerhs=True):
         """one RK4 step of the coupled system (lists of 30 indices)"""
         L = lib()
         _sig(L, "bamhip_z4c_grhd_rk4_step", C.c_int,
              [C.c_void_p, C.POINTER(Z4cGrhdStruct), C.c_double, IP, IP, IP, IP, C.c_int])
         S, keep = self._z4c_grhd(z4, c2p, hrsc, adm_g, adm_K, prim, madm, status, iters, method, mlp, polish_max,
-                                 fallbacks, pgh)
+                                 fallbacks, pgh, atm_prerhs)
         check(L.bamhip_z4c_grhd_rk4_step(self.h, C.byref(S), dt, _iarr(uc), _iarr(up), _iarr(r), _iarr(v),
                                          1 if algcon else 0), "bamhip_z4c_grhd_rk4_step")
 
@@ -509,23 +534,69 @@ class FmrHierarchyStruct(C.Structure):
                 ("order_RP", C.c_int), ("timeinterpolation", C.c_int), ("pgh", C.c_int),
                 ("npunc", C.c_int), ("punc", C.c_void_p), ("punc_flags", C.c_int),
                 ("move_nxyz", C.c_int), ("move_lcube", C.c_int),
-                ("nbuffer", C.c_int), ("fp", C.c_int), ("fr", C.c_int)]
+                ("nbuffer", C.c_int), ("fp", C.c_int), ("fr", C.c_int),
+                ("nx", C.c_int), ("nz", C.c_int), ("n_amr_nxyz", C.c_int), ("amr_nxyz", IP),
+                ("addextra", C.c_int)]
 
 
-def find_nested_bboxes(points, h, nxyz):
-    """bamhip_find_nested_bboxes (host only): the coarse bounding boxes
-    (nboxes, 6) and index boxes around the points (np, 3) on spacing h"""
+class NestedSizesStruct(C.Structure):
+    _fields_ = [("nx", C.c_int), ("nz", C.c_int), ("move_nxyz", C.c_int), ("move_lcube", C.c_int),
+                ("nlist", C.c_int), ("list", IP), ("nbuffer", C.c_int), ("addextra", C.c_int)]
+
+
+class NestedSizes:
+    """the box-size parameters of find_nested_bboxes: nx, nz of the coarsest
+    grid, amr_move_nxyz, amr_move_lcube, the amr_nxyz list, amr_nbuffer and
+    amr_buffer_addextra (the reference default: yes)"""
+
+    def __init__(self, nx, nz=None, move_nxyz=0, move_lcube=0, nxyz_list=(), nbuffer=6, addextra=True):
+        self.nx, self.nz = int(nx), int(nx if nz is None else nz)
+        self.move_nxyz, self.move_lcube = int(move_nxyz), int(move_lcube)
+        self.nxyz_list = [int(x) for x in nxyz_list]
+        self.nbuffer, self.addextra = int(nbuffer), bool(addextra)
+        self._list = (C.c_int * max(len(self.nxyz_list), 1))(*self.nxyz_list)
+        self.s = NestedSizesStruct(self.nx, self.nz, self.move_nxyz, self.move_lcube, len(self.nxyz_list),
+                                   C.cast(self._list, IP), self.nbuffer, 1 if self.addextra else 0)
+
+
+def nested_box_size(sizes, lfine):
+    """bamhip_nested_box_size (host only): coarse points per direction of the
+    boxes of fine level lfine"""
     L = lib()
-    _sig(L, "bamhip_find_nested_bboxes", C.c_int, [C.c_int, DP, DP, C.c_int, IP, DP, IP])
+    _sig(L, "bamhip_nested_box_size", C.c_int, [C.POINTER(NestedSizesStruct), C.c_int, IP])
+    n = (C.c_int * 3)()
+    check(L.bamhip_nested_box_size(C.byref(sizes.s), int(lfine), n), "bamhip_nested_box_size")
+    return list(n)
+
+
+def find_nested_bboxes(sizes, lfine, points, h):
+    """bamhip_find_nested_bboxes (host only): the coarse bounding boxes
+    (nboxes, 6) and index boxes of fine level lfine around the points (np, 3)
+    on the coarse spacing h"""
+    L = lib()
+    _sig(L, "bamhip_find_nested_bboxes", C.c_int,
+         [C.POINTER(NestedSizesStruct), C.c_int, C.c_int, DP, DP, IP, DP, IP])
     p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
     hh = np.ascontiguousarray(np.broadcast_to(np.asarray(h, dtype=np.float64), (3,)))
     nb = C.c_int()
     bb = (C.c_double * 60)()
     ib = (C.c_int * 60)()
-    check(L.bamhip_find_nested_bboxes(p.shape[0], p.ctypes.data_as(DP), hh.ctypes.data_as(DP), int(nxyz),
-                                      C.byref(nb), bb, ib), "bamhip_find_nested_bboxes")
+    check(L.bamhip_find_nested_bboxes(C.byref(sizes.s), int(lfine), p.shape[0], p.ctypes.data_as(DP),
+                                      hh.ctypes.data_as(DP), C.byref(nb), bb, ib), "bamhip_find_nested_bboxes")
     n = nb.value
     return np.array(bb[:6 * n]).reshape(n, 6), np.array(ib[:6 * n]).reshape(n, 6)
+
+
+def level_boxes_of_bboxes(bb, ib, h):
+    """make_level_bboxes (box.c:465-560): the fine boxes of coarse bounding
+    boxes (2 (cells + 1) points per direction, the first point half a fine
+    spacing inside) as (m, n, o, x0, y0, z0, dx) tuples"""
+    out = []
+    hh = np.broadcast_to(np.asarray(h, dtype=np.float64), (3,))
+    for b in range(bb.shape[0]):
+        m, n_, o = (2 * (int(ib[b, 2 * d + 1]) + 1) for d in range(3))
+        out.append((m, n_, o, bb[b, 0] - hh[0] / 4, bb[b, 2] - hh[1] / 4, bb[b, 4] - hh[2] / 4, hh[0] / 2))
+    return out
 
 
 def tov_h(hc, K=100.0, Gamma=2.0, npts=100000):
@@ -555,7 +626,7 @@ class Hierarchy:
     params: one Z4cParams per level."""
 
     def __init__(self, levels, params, dt0, u, up, upp, r, v, order_RP=4, timeinterpolation=3, pgh=0,
-                 punctures=None, pin_z=False, move_nxyz=0, move_lcube=0, rp_flags=(3, -1, -1)):
+                 punctures=None, pin_z=False, sizes=None, rp_flags=(3, -1, -1)):
         L = lib()
         hp = C.POINTER(FmrHierarchyStruct)
         _sig(L, "bamhip_advance_levelandsublevels", C.c_int, [hp, C.c_int])
@@ -583,7 +654,14 @@ class Hierarchy:
         self.punctures = punctures
         if punctures is not None:
             s.npunc, s.punc, s.punc_flags = punctures.shape[0], punctures.ptr, 1 if pin_z else 0
-        s.move_nxyz, s.move_lcube = move_nxyz, move_lcube
+        # sizes: NestedSizes of the moving boxes (None: the boxes stay)
+        self.sizes = sizes
+        if sizes is not None:
+            assert sizes.move_nxyz > 0, "moving boxes need amr_move_nxyz > 0"
+            assert sizes.nbuffer == rp_flags[0], "amr_nbuffer of the sizes and of the flags differ"
+            s.move_nxyz, s.move_lcube = sizes.move_nxyz, sizes.move_lcube
+            s.nx, s.nz, s.n_amr_nxyz, s.amr_nxyz = sizes.nx, sizes.nz, len(sizes.nxyz_list), C.cast(sizes._list, IP)
+            s.addextra = 1 if sizes.addextra else 0
         s.nbuffer, s.fp, s.fr = rp_flags
         self.s = s
 
@@ -656,7 +734,7 @@ class CheckpointStruct(C.Structure):
     _fields_ = [("lmin", C.c_int), ("lmax", C.c_int), ("levels", C.POINTER(C.c_void_p)), ("iteration", IP),
                 ("time", DP), ("nparams", C.c_int), ("param_names", C.POINTER(C.c_char_p)),
                 ("param_values", C.POINTER(C.c_char_p)), ("nvars", C.c_int), ("vars", IP),
-                ("var_names", C.POINTER(C.c_char_p))]
+                ("var_names", C.POINTER(C.c_char_p)), ("dt", DP)]
 
 
 def _checkpoint_struct(levels, iterations, times, vars, names, params, lmin):
@@ -679,12 +757,16 @@ def checkpoint_write(path, levels, iterations, times, vars, names, params=None, 
     check(L.bamhip_checkpoint_write(str(path).encode(), C.byref(S)), "bamhip_checkpoint_write")
 
 
-def checkpoint_read(path, levels, vars, names, lmin=0):
-    """checkpoint_read into device levels; returns (iterations, times) per level"""
+def checkpoint_read(path, levels, vars, names, lmin=0, dt=None):
+    """checkpoint_read into device levels; returns (iterations, times) per
+    level; with dt (per level) the times are iteration * dt"""
     L = lib()
     _sig(L, "bamhip_checkpoint_read", C.c_int, [C.c_char_p, C.POINTER(CheckpointStruct), IP, DP])
     n = len(levels)
     S, keep = _checkpoint_struct(levels, [0] * n, [0.0] * n, vars, names, {}, lmin)
+    if dt is not None:
+        keep["dt"] = (C.c_double * n)(*dt)
+        S.dt = C.cast(keep["dt"], DP)
     it, tm = (C.c_int * n)(), (C.c_double * n)()
     check(L.bamhip_checkpoint_read(str(path).encode(), C.byref(S), it, tm), "bamhip_checkpoint_read")
     return list(it), list(tm)
@@ -709,7 +791,7 @@ class Z4cGrhdStruct(C.Structure):
     _fields_ = [("z4", C.POINTER(Z4cParams)), ("c2p", C.POINTER(C2PParams)), ("hrsc", C.POINTER(HrscParams)),
                 ("c2p_method", C.c_int), ("mlp", C.c_void_p), ("polish_max", C.c_int), ("fallbacks", C.c_void_p),
                 ("pgh", C.c_int), ("adm_g", IP), ("adm_K", IP), ("prim", IP), ("madm", IP),
-                ("status", C.c_void_p), ("iters", C.c_void_p)]
+                ("status", C.c_void_p), ("iters", C.c_void_p), ("atm_prerhs", C.c_int)]
 
 
 def c2p_params(eos="ideal", gamma=2.0, K=100.0, rhoatm=1e-20, fatm=100.0, vmax=0.999, errormax=1e-10,

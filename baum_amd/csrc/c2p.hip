@@ -535,6 +535,43 @@ __global__ void p2c_kernel(C2PK P, long n, const double *__restrict__ prim5, con
   }
 }
 
+// Cold static atmosphere on the evolved conserved variables, grhd_atm_set_u
+// (src/projects/grhd/grhd_atm.c:217-262) as called by grhd_atm_set_prerhs
+// (:268-278) on u_p: where rho < f_atm rho_atm, D = sqrt(detg) rho_atm,
+// S_i = 0, tau = sqrt(detg) (rho_atm eps_atm), with detg raised to DETGMIN
+// (grhd.h: 1e-5) in the primitives and sqrt(detg) = 1 if not finite.  A
+// point still non-finite afterwards is zeroed (detg = 1) and raises *flag
+// (the reference's errorexit "nans not cured by setting atmosphere").
+__global__ void atm_set_u_kernel(C2PArrays A, C2PK P, long n, int *flag) {
+  const double DETGMIN = 1e-5;
+  const double rhoatmlevel = P.rhoatm * P.fatm;
+  const double Datmlevel = P.rhoatm;
+  const double Tatmlevel = P.rhoatm * P.epslatm;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    double D = A.cons[0][q], T = A.cons[1][q], Sx = A.cons[2][q], Sy = A.cons[3][q], Sz = A.cons[4][q];
+    double detg = A.prim[7][q];
+    if (A.prim[0][q] < rhoatmlevel) {
+      detg = detg > DETGMIN ? detg : DETGMIN;
+      double s = sqrt(detg);
+      if (!isfinite(s)) s = 1.;
+      D = s * Datmlevel;
+      Sx = Sy = Sz = 0.;
+      T = s * Tatmlevel;
+      A.cons[0][q] = D;
+      A.cons[1][q] = T;
+      A.cons[2][q] = Sx;
+      A.cons[3][q] = Sy;
+      A.cons[4][q] = Sz;
+      A.prim[7][q] = detg;
+    }
+    if (!isfinite(D) || !isfinite(Sx) || !isfinite(Sy) || !isfinite(Sz) || !isfinite(T) || !isfinite(detg)) {
+      for (int k = 0; k < 5; k++) A.cons[k][q] = 0.;
+      A.prim[7][q] = 1.;
+      *flag = 1;
+    }
+  }
+}
+
 static C2PK make_c2pk(const bamhip_c2p_params *p) {
   C2PK K;
   K.eos = p->eos;
@@ -663,6 +700,39 @@ extern "C" int bamhip_p2c_batch(const bamhip_c2p_params *par, long n, const doub
   if (blocks > 65536) blocks = 65536;
   p2c_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(make_c2pk(par), n, prim5, metric, cons);
   BH_HIP(hipGetLastError());
+  return BAMHIP_OK;
+}
+
+__global__ void atm_flag_set(int *p, int v) { *p = v; }
+
+extern "C" int bamhip_grhd_atm_set_prerhs(bamhip_level *lev, const bamhip_c2p_params *par, const int *upre5,
+                                          const int *prim8) {
+  if (!lev || !par || !upre5 || !prim8) return set_error(BAMHIP_EINVAL, "bamhip_grhd_atm_set_prerhs: NULL argument");
+  C2PArrays A = {};
+  double *tmp[8];
+  int e;
+  if ((e = gather_ptrs(lev, upre5, 5, tmp, "bamhip_grhd_atm_set_prerhs(upre)"))) return e;
+  for (int k = 0; k < 5; k++) A.cons[k] = tmp[k];
+  if ((e = gather_ptrs(lev, prim8, 8, tmp, "bamhip_grhd_atm_set_prerhs(prim)"))) return e;
+  for (int k = 0; k < 8; k++) A.prim[k] = tmp[k];
+  const long n = lev->npoints;
+  if (n == 0) return BAMHIP_OK;
+  if (lev->check_mode) atm_flag_set<<<1, 1, 0, stream()>>>(lev->d_flag, 0);
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  {
+    TimerScope t("c2p");
+    atm_set_u_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(A, make_c2pk(par), n, lev->d_flag);
+    BH_HIP(hipGetLastError());
+  }
+  if (lev->check_mode) {
+    int h = 0;
+    BH_HIP(hipMemcpyAsync(&h, lev->d_flag, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    BH_HIP(hipStreamSynchronize(stream()));
+    if (h)
+      return set_error(BAMHIP_ENONFINITE, "bamhip_grhd_atm_set_prerhs: nans not cured by setting atmosphere (level %d)",
+                       lev->l);
+  }
   return BAMHIP_OK;
 }
 

@@ -99,6 +99,10 @@ int find_parent(const bamhip_level *lc, const DevBox &F) {
   return -1;
 }
 
+__global__ void or_flag_kernel(int *dst, const int *src) {
+  if (*src) *dst = 1;
+}
+
 int link_level(bamhip_level *lc, bamhip_level *lf, const bamhip_fmr_hierarchy *H, const char *what) {
   std::vector<int> parent(lf->nboxes);
   for (int b = 0; b < lf->nboxes; b++) {
@@ -123,22 +127,47 @@ using namespace bamhip;
 
 extern "C" {
 
-int bamhip_find_nested_bboxes(int np, const double *pos, const double *h, int nxyz, int *nboxes, double *bbox,
-                              int *ibbox) {
+int bamhip_nested_box_size(const bamhip_nested_sizes *S, int lfine, int *n) {
+  const char *what = "bamhip_nested_box_size";
+  if (!S || !n || lfine < 1 || S->nlist < 0 || (S->nlist > 0 && !S->list) || S->nbuffer < 0)
+    return set_error(BAMHIP_EINVAL, "%s: bad arguments", what);
+  // the coarsest grid's size; ny takes nx, as the reference does
+  n[0] = n[1] = S->nx;
+  n[2] = S->nz;
+  if (n[0] % 2 || n[1] % 2 || n[2] % 2) return set_error(BAMHIP_EINVAL, "%s: nested boxes need even nx, ny, nz", what);
+  if (lfine > S->move_lcube && S->move_nxyz) n[0] = n[1] = n[2] = S->move_nxyz;
+  if (S->nlist) {
+    const int j = (S->nlist - 1 > lfine) ? lfine : S->nlist - 1;
+    n[0] = n[1] = n[2] = S->list[j];
+    if (n[0] % 4) return set_error(BAMHIP_EINVAL, "%s: box sizes in amr_nxyz have to be multiples of 4", what);
+  }
+  for (int i = 0; i < 3; i++) {
+    if (S->addextra) n[i] += 2 * S->nbuffer;
+    // factor two nesting, rounded up to an even number of coarse points
+    n[i] = n[i] / 2 + (n[i] / 2) % 2;
+    if (n[i] < 2) return set_error(BAMHIP_EINVAL, "%s: box of %d coarse points on level %d", what, n[i], lfine);
+  }
+  return BAMHIP_OK;
+}
+
+int bamhip_find_nested_bboxes(const bamhip_nested_sizes *S, int lfine, int np, const double *pos, const double *h,
+                              int *nboxes, double *bbox, int *ibbox) {
   const char *what = "bamhip_find_nested_bboxes";
   if (np < 1 || np > 10 || !pos || !h || !nboxes || !bbox || !ibbox)
     return set_error(BAMHIP_EINVAL, "%s: bad arguments (np = %d)", what, np);
-  if (nxyz <= 0 || nxyz % 2) return set_error(BAMHIP_EINVAL, "%s: nested boxes need even nx, ny, nz", what);
   for (int i = 0; i < 3; i++)
     if (!(h[i] > 0)) return set_error(BAMHIP_EINVAL, "%s: spacing %d is not positive", what, i);
-  // factor two nesting, rounded up to an even number of coarse points
-  const int n = nxyz / 2 + (nxyz / 2) % 2;
+  int n[3];
+  if (int e = bamhip_nested_box_size(S, lfine, n)) return e;
+  // levels up to amr_move_lcube: one centred cube, every point at the origin
+  const bool cube = lfine <= S->move_lcube;
   double bb[10][6];
   for (int j = 0; j < np; j++)
     for (int i = 0; i < 3; i++) {
-      const double p = ((int)(pos[3 * j + i] / h[i])) * h[i];  // truncation: symmetric about the origin
-      bb[j][2 * i] = p - (n - 1) * h[i] / 2;
-      bb[j][2 * i + 1] = p + (n - 1) * h[i] / 2;
+      const double p0 = cube ? 0.0 : pos[3 * j + i];
+      const double p = ((int)(p0 / h[i])) * h[i];  // truncation: symmetric about the origin
+      bb[j][2 * i] = p - (n[i] - 1) * h[i] / 2;
+      bb[j][2 * i + 1] = p + (n[i] - 1) * h[i] / 2;
     }
   int nb = np;
   for (int i = 0; i < nb - 1; i++)
@@ -245,7 +274,16 @@ int bamhip_move_box(const bamhip_fmr_hierarchy *H, int l, int *moved) {
   if (l - 2 >= 0 && !dless(time_of(l - 1), time_of(l - 2))) return BAMHIP_OK;
   const int np = H->npunc;
   if (np < 1 || np > 10 || !H->punc) return set_error(BAMHIP_EINVAL, "%s: needs 1 to 10 punctures", what);
-  std::vector<double> punc(3 * np), origin(3 * np, 0.0);
+  bamhip_nested_sizes S;
+  S.nx = H->nx;
+  S.nz = H->nz;
+  S.move_nxyz = H->move_nxyz;
+  S.move_lcube = H->move_lcube;
+  S.nlist = H->n_amr_nxyz;
+  S.list = H->amr_nxyz;
+  S.nbuffer = H->nbuffer;
+  S.addextra = H->addextra;
+  std::vector<double> punc(3 * np);
   hipStream_t st = stream();
   BH_HIP(hipMemcpyAsync(punc.data(), H->punc, sizeof(double) * 3 * np, hipMemcpyDeviceToHost, st));
   BH_HIP(hipStreamSynchronize(st));
@@ -257,9 +295,7 @@ int bamhip_move_box(const bamhip_fmr_hierarchy *H, int l, int *moved) {
     int nb = 0;
     double bb[60];
     int ib[60];
-    if (int e = bamhip_find_nested_bboxes(np, l <= H->move_lcube ? origin.data() : punc.data(), h, H->move_nxyz,
-                                          &nb, bb, ib))
-      return e;
+    if (int e = bamhip_find_nested_bboxes(&S, l, np, punc.data(), h, &nb, bb, ib)) return e;
     bool same = nb == lold->nboxes;
     for (int b = 0; same && b < nb; b++) {
       double cb[6];
@@ -312,7 +348,10 @@ int bamhip_move_box(const bamhip_fmr_hierarchy *H, int l, int *moved) {
       bamhip_level_free(lnew);
       return e;
     }
-    // the old level's storage goes once the copy has run
+    // a non-finite value the old level's deferred check recorded stays
+    // reported; the old level's storage goes once the copy has run
+    or_flag_kernel<<<1, 1, 0, st>>>(lnew->d_flag, lold->d_flag);
+    BH_HIP(hipGetLastError());
     BH_HIP(hipStreamSynchronize(st));
     bamhip_level_free(lold);
     H->levels[l] = lnew;

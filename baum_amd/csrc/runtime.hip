@@ -356,6 +356,12 @@ int bamhip_level_set_check(bamhip_level *lev, int mode) {
   return BAMHIP_OK;
 }
 
+int bamhip_level_get_check(const bamhip_level *lev, int *mode) {
+  if (!lev || !mode) return set_error(BAMHIP_EINVAL, "bamhip_level_get_check: NULL argument");
+  *mode = lev->check_mode;
+  return BAMHIP_OK;
+}
+
 int bamhip_level_nonfinite(bamhip_level *lev, int reset, int *flag) {
   if (!lev) return set_error(BAMHIP_EINVAL, "bamhip_level_nonfinite: NULL level");
   if (flag) {

@@ -113,6 +113,10 @@ int bamhip_checkpoint_read(const char *path, const bamhip_checkpoint *C, int *it
       }
     }
   }
+  /* the header's time is for human inspection (%g); the reference restarts
+   * with time = iteration * dt (checkpoint_read.c:117-124) */
+  if (time && iteration && C->dt)
+    for (int l = C->lmin; l <= C->lmax; l++) time[l - C->lmin] = iteration[l - C->lmin] * C->dt[l - C->lmin];
   /* variables */
   while (fgets(line, sizeof line, fp))
     if (!strncmp(line, "$BEGIN_variables:", 17)) break;

@@ -115,7 +115,8 @@ int bamhip_z4c_algcon_list(bamhip_level *lev, const bamhip_z4c_params *par, cons
 int bamhip_z4c_rk4_step(bamhip_level *lev, const bamhip_z4c_params *par, double dt, const int *uc, const int *up,
                         const int *r, const int *v, int pgh, int algcon)
 {
-  int e, flag = 0;
+  int e, flag = 0, mode;
+  if ((e = bamhip_level_get_check(lev, &mode))) return e;
   if ((e = bamhip_level_set_check(lev, 0))) return e;
   if ((e = bamhip_level_nonfinite(lev, 1, 0))) goto out;
   if ((e = bamhip_z4c_rk4_stages(lev, par, dt, uc, up, r, v, pgh))) goto out;
@@ -123,6 +124,6 @@ int bamhip_z4c_rk4_step(bamhip_level *lev, const bamhip_z4c_params *par, double 
   if ((e = bamhip_level_nonfinite(lev, 0, &flag))) goto out;
   if (flag) e = BAMHIP_ENONFINITE;
 out:
-  bamhip_level_set_check(lev, 1);
+  bamhip_level_set_check(lev, mode);
   return e;
 }

@@ -6,6 +6,7 @@
  *   rhs_geometry_adm      z4_to_adm_vl (z4.c:93-124)   bamhip_z4c_to_adm
  *   rhs_matter            matter_rhs (matter.c:373-482):
  *     MATTER.c2p            (hybrid p-root etc.)       bamhip_grhd_c2p(_learned)
+ *     MATTER.set_atm_prerhs grhd_atm_set_prerhs        bamhip_grhd_atm_set_prerhs
  *     matter_rhs_prepare    (matter.c:323-352)         bamhip_vladd
  *     MATTER.sources        grhd_sources               bamhip_grhd_sources
  *     fluxes                matter_fluxes              bamhip_grhd_fluxes
@@ -43,6 +44,9 @@ int bamhip_z4c_grhd_rhs(bamhip_level *lev, const bamhip_z4c_grhd *S, const int *
   else
     e = bamhip_grhd_c2p(lev, S->c2p, S->c2p_method, qc, S->adm_g, S->prim, S->status, S->iters);
   if (e) return e;
+  /* (MATTER.USEATM && MATTER.USEATM_PRERHS): the atmosphere of u_p from the
+   * new primitives, before matter_rhs_prepare copies u_p (c != 0) */
+  if (S->atm_prerhs && qp && (e = bamhip_grhd_atm_set_prerhs(lev, S->c2p, qp, S->prim))) return e;
   if ((e = bamhip_vladd(lev, 5, qn, c != 0. ? 1. : 0., c != 0. ? qp : qc, 0., qc))) return e;
   if ((e = bamhip_grhd_sources(lev, S->hrsc, S->c2p->rhoatm, S->z4->order_centered, qn, c, metric, S->adm_K,
                                S->prim, S->madm)))
@@ -54,18 +58,21 @@ int bamhip_z4c_grhd_rhs(bamhip_level *lev, const bamhip_z4c_grhd *S, const int *
 }
 
 /* One RK4 step of the coupled system: evolve_rk_rk4 (src/utility/evolve/
- * rk.c:220-257) over the 30 evolved fields with the RHS above (c = 0 per
- * stage), then z4_algcon_wrapper on the Z4c fields (z4.c:49-71).  The RHS's
- * non-finite check is deferred to the end of the step, as in
- * bamhip_z4c_rk4_step.  Lists hold 30 indices (25 Z4c, then D, tau, S_i). */
+ * rk.c:220-257) over the 30 evolved fields with the RHS above, called as
+ * evolve_rhs(r, u_p, 0.0, v) so that the atmosphere reset acts on u_p, then
+ * z4_algcon_wrapper on the Z4c fields (z4.c:49-71).  The RHS's non-finite
+ * check is deferred to the end of the step, as in bamhip_z4c_rk4_step, and
+ * the caller's check mode is restored.  Lists hold 30 indices (25 Z4c, then
+ * D, tau, S_i). */
 int bamhip_z4c_grhd_rk4_step(bamhip_level *lev, const bamhip_z4c_grhd *S, double dt, const int *uc, const int *up,
                              const int *r, const int *v, int algcon)
 {
   const int nz = BAMHIP_Z4C_NVARS;
   const double wc[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
   const double wv[4] = {dt / 2, dt / 2, dt, 0.0};
-  int e, s, flag = 0;
+  int e, s, flag = 0, mode;
   if (!lev || !S || !uc || !up || !r || !v) return BAMHIP_EINVAL;
+  if ((e = bamhip_level_get_check(lev, &mode))) return e;
   if ((e = bamhip_level_set_check(lev, 0))) return e;
   if ((e = bamhip_level_nonfinite(lev, 1, 0))) goto out;
   /* u_p = u_c */
@@ -73,7 +80,7 @@ int bamhip_z4c_grhd_rk4_step(bamhip_level *lev, const bamhip_z4c_grhd *S, double
   if ((e = bamhip_vladd(lev, 5, up + nz, 1.0, uc + nz, 0.0, 0))) goto out;
   for (s = 0; s < 4; s++) {
     const int *src = (s == 0) ? uc : v;
-    if ((e = bamhip_z4c_grhd_rhs(lev, S, r, 0, 0.0, src))) goto out;
+    if ((e = bamhip_z4c_grhd_rhs(lev, S, r, up, 0.0, src))) goto out;
     if ((e = bamhip_rk4_stage_update(lev, nz, uc, up, r, v, wc[s], wv[s]))) goto out;
     if ((e = bamhip_rk4_stage_update(lev, 5, uc + nz, up + nz, r + nz, v + nz, wc[s], wv[s]))) goto out;
   }
@@ -81,6 +88,6 @@ int bamhip_z4c_grhd_rk4_step(bamhip_level *lev, const bamhip_z4c_grhd *S, double
   if ((e = bamhip_level_nonfinite(lev, 0, &flag))) goto out;
   if (flag) e = BAMHIP_ENONFINITE;
 out:
-  bamhip_level_set_check(lev, 1);
+  bamhip_level_set_check(lev, mode);
   return e;
 }

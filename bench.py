@@ -383,9 +383,11 @@ def run_move():
     nvar, nxyz, nlev = 127, 96, 5
     U, UP, UPP, R, V = (list(range(25 * k, 25 * k + 25)) for k in range(5))
     levels = [B.Level([B.make_box(192, 192, 192, -95.5, -95.5, -95.5, 1.0, bflag=(B.PHYBOUND,) * 6)], nvar)]
+    # amr_move_nxyz 96 with amr_buffer_addextra = no: 96^3 fine boxes
+    sizes = B.NestedSizes(nx=192, move_nxyz=nxyz, nbuffer=6, addextra=False)
     for l in range(1, nlev):
         hc = levels[-1].boxes[0].dx
-        bb, ib = B.find_nested_bboxes([[0.0, 0.0, 0.0]], hc, nxyz)
+        bb, ib = B.find_nested_bboxes(sizes, l, [[0.0, 0.0, 0.0]], hc)
         d = hc / 2
         b = B.make_box(2 * (ib[0][1] + 1), 2 * (ib[0][3] + 1), 2 * (ib[0][5] + 1), bb[0][0] - d / 2, bb[0][2] - d / 2,
                        bb[0][4] - d / 2, d, bflag=(B.REFBOUND,) * 6)
@@ -397,7 +399,7 @@ def run_move():
     for l in range(1, nlev):
         B.set_rp_flags(levels[l - 1], levels[l], [0], 6, 125, 126)
     punc = B.DeviceArray.from_host(np.zeros((1, 3)))
-    H = B.Hierarchy(levels, [B.z4c_params()] * nlev, 0.25, U, UP, UPP, R, V, punctures=punc, move_nxyz=nxyz,
+    H = B.Hierarchy(levels, [B.z4c_params()] * nlev, 0.25, U, UP, UPP, R, V, punctures=punc, sizes=sizes,
                     rp_flags=(6, 125, 126))
     res = []
     for shift in (1.0, 2.0):

@@ -178,13 +178,16 @@ int bamhip_rk4_stage_update(bamhip_level *lev, int n, const int *uc, const int *
  * 0 = no synchronisation, the flag accumulates until bamhip_level_nonfinite
  * reads (and optionally resets) it */
 int bamhip_level_set_check(bamhip_level *lev, int mode);
+int bamhip_level_get_check(const bamhip_level *lev, int *mode);
 int bamhip_level_nonfinite(bamhip_level *lev, int reset, int *flag);
 
 /* Replaces evolve_rk_rk4 (src/utility/evolve/rk.c:220-257) for the Z4c list:
  * one full RK4 step of size dt on the level with the RHS and boundary above,
  * using fused stage kernels.  uc: state (updated), up: saved state u_p,
  * r, v: stage registers.  Algebraic constraints (z4_algcon_wrapper) are
- * applied afterwards when algcon != 0.  pgh: periodic layers (0 = none). */
+ * applied afterwards when algcon != 0.  pgh: periodic layers (0 = none).
+ * The non-finite check runs once at the end of the step; the level's check
+ * mode is restored on return. */
 int bamhip_z4c_rk4_step(bamhip_level *lev, const bamhip_z4c_params *par, double dt,
                         const int *uc, const int *up, const int *r, const int *v,
                         int pgh, int algcon);
@@ -254,6 +257,21 @@ int bamhip_p2c_batch(const bamhip_c2p_params *par, long n, const double *prim5, 
  * status / iters: device arrays of npoints ints. */
 int bamhip_grhd_c2p(bamhip_level *lev, const bamhip_c2p_params *par, int method, const int *cons5,
                     const int *metric6, const int *prim8, int *status, int *iters);
+
+/* Replaces MATTER.set_atm_prerhs = grhd_atm_set_prerhs(unew, upre, c, ucur,
+ * primVars) (src/projects/grhd/grhd_atm.c:268-278, registered for the
+ * ColdStatic atmosphere at grhd.c:99-110 and called after the c2p at
+ * matter.c:414-419) with grhd_use_atmosphere_mask = no (its default):
+ * grhd_atm_set_u(upre, primVars) (grhd_atm.c:217-262) at every point of the
+ * level.  Where rho < fatm * rhoatm: detg = max(detg, 1e-5) in prim8[7],
+ * D = sqrt(detg) * rhoatm, S_i = 0, tau = sqrt(detg) * (rhoatm * epslatm)
+ * (sqrt(detg) = 1 if not finite).  upre5: D, tau, S_x, S_y, S_z; prim8 as
+ * bamhip_grhd_c2p (rho read, detg read and written).  A point non-finite
+ * after the reset is zeroed (detg = 1) and raises the level's non-finite
+ * flag (the reference's errorexit): in check mode 1 the call synchronises
+ * and returns BAMHIP_ENONFINITE, in mode 0 the flag is deferred. */
+int bamhip_grhd_atm_set_prerhs(bamhip_level *lev, const bamhip_c2p_params *par, const int *upre5,
+                               const int *prim8);
 
 /* ----------------------------------------------- learned pressure guess */
 /* A 3 -> 64 -> 64 -> 1 ReLU network predicting log(p / ((Gamma-1) Delta)),
@@ -377,6 +395,13 @@ typedef struct {
   int move_nxyz;           /* 0: boxes stay where they are */
   int move_lcube;
   int nbuffer, fp, fr;
+  /* box sizes of the moved levels (bamhip_nested_sizes): nx, nz of the
+   * coarsest grid, the amr_nxyz list (n_amr_nxyz entries, may be 0) and
+   * amr_buffer_addextra */
+  int nx, nz;
+  int n_amr_nxyz;
+  const int *amr_nxyz;
+  int addextra;
 } bamhip_fmr_hierarchy;
 
 /* Replaces advance_levelandsublevels(g, l) (src/main/main/main.c:397-453)
@@ -394,15 +419,32 @@ int bamhip_fmr_set_check(const bamhip_fmr_hierarchy *H, int mode);
 int bamhip_fmr_nonfinite(const bamhip_fmr_hierarchy *H, int reset, int *flag);
 
 /* ---------------------------------------------------------- moving boxes */
-/* find_nested_bboxes (src/main/amr/move.c:121-465) without symmetries or
- * forced motion: around each of the np (<= 10) points pos[3*j..] snapped
- * to the coarse spacing h[3] by truncation, a box of n = nxyz/2 rounded up
- * to even coarse points per direction; overlapping boxes are joined in the
- * reference's pairwise order.  Out: *nboxes, bbox[6*b] (coarse coordinates
- * of the first and last point per direction), ibbox[6*b] (0, cells, ...).
- * Host only.  nxyz must be even and positive. */
-int bamhip_find_nested_bboxes(int np, const double *pos, const double *h, int nxyz, int *nboxes, double *bbox,
-                              int *ibbox);
+/* The box-size parameters find_nested_bboxes reads (src/main/amr/
+ * move.c:233-267; defaults in bam_amr.c:41-69, bam_main.c). */
+typedef struct {
+  int nx, nz;      /* "nx", "nz" of the coarsest grid (n_y takes nx, as the reference does) */
+  int move_nxyz;   /* amr_move_nxyz: size of the moving cubes (0: nx) */
+  int move_lcube;  /* amr_move_lcube: fine levels up to it are cubes around the origin */
+  int nlist;       /* entries of amr_nxyz (0: none) */
+  const int *list; /* amr_nxyz: per-level sizes, multiples of 4 */
+  int nbuffer;     /* amr_nbuffer */
+  int addextra;    /* amr_buffer_addextra (the reference default: yes = 1) */
+} bamhip_nested_sizes;
+/* n[3] coarse points per direction of the boxes of fine level lfine
+ * (move.c:233-267): nx (nz) of the coarsest grid, amr_move_nxyz above
+ * amr_move_lcube, else the amr_nxyz entry min(nlist - 1, lfine), plus
+ * 2 nbuffer with addextra, then halved and rounded up to even.  Host only. */
+int bamhip_nested_box_size(const bamhip_nested_sizes *S, int lfine, int *n);
+/* find_nested_bboxes (move.c:121-465) for fine level lfine, without
+ * symmetries or forced motion: around each of the np (<= 10) points pos[3*j..]
+ * (all at the origin when lfine <= move_lcube) snapped to the coarse
+ * spacing h[3] by truncation, a box of bamhip_nested_box_size points;
+ * overlapping boxes are joined in the reference's pairwise order.  Out:
+ * *nboxes, bbox[6*b] (coarse coordinates of the first and last point per
+ * direction), ibbox[6*b] (0, cells, ...).  The same function lays out the
+ * initial grid (make_level_nested_boxes, box.c:437-450).  Host only. */
+int bamhip_find_nested_bboxes(const bamhip_nested_sizes *S, int lfine, int np, const double *pos, const double *h,
+                              int *nboxes, double *bbox, int *ibbox);
 /* move_overlap_nonparallel (move.c:589-676) with copyall: for every pair
  * (old box, new box) in that order, copy the n variables at the points the
  * two boxes share (index ranges of box_overlap_range, move.c:487-533);
@@ -413,7 +455,9 @@ int bamhip_level_copy_overlap(bamhip_level *lold, bamhip_level *lnew, int n, con
  * unless move_nxyz > 0, 1 <= l <= lmax and level l-1 is the coarsest of its
  * time (time[l-1] < time[l-2]); then from l to lmax the nested boxes of
  * each level are found on its parent around the punctures (the origin up
- * to move_lcube); a level whose boxes differ is replaced by a new one
+ * to move_lcube) with the hierarchy's box sizes (nx, nz, move_nxyz,
+ * amr_nxyz, nbuffer, addextra); a level whose boxes differ is replaced by a
+ * new one
  * (make_level_bboxes, box.c:465-560: 2*(cells+1) points per direction,
  * first point half a fine spacing inside the coarse box, REFBOUND faces)
  * with the old level's enabled variables, filled by order_RP prolongation
@@ -421,7 +465,8 @@ int bamhip_level_copy_overlap(bamhip_level *lold, bamhip_level *lnew, int n, con
  * upp, then by the copy of the overlap with the old level (the reference's
  * global and local methods give these values); a level kept in place is
  * linked again to a replaced parent.  The box comparison is with the
- * level's present boxes.  *moved (may be NULL) = 1 if a level moved. */
+ * level's present boxes.  A non-finite flag of a replaced level passes to
+ * its successor.  *moved (may be NULL) = 1 if a level moved. */
 int bamhip_move_box(const bamhip_fmr_hierarchy *H, int l, int *moved);
 
 /* RK4 stages without algcon and without the non-finite check, and algcon on
@@ -540,6 +585,7 @@ int bamhip_z4c_to_adm(bamhip_level *lev, const int *ucur, double chipsipower, co
  * evolve.c:37-111) for physics z4 + matter + grhd as one call:
  * bamhip_z4c_rhs, bamhip_z4c_to_adm, the c2p of c2p_method (BAMHIP_C2P_*
  * on the hybrid/plain/bracketed solvers, or BAMHIP_C2P_LEARNED with mlp),
+ * the atmosphere reset of upre (atm_prerhs, upre != NULL),
  * matter_rhs_prepare, bamhip_grhd_sources, bamhip_grhd_fluxes,
  * bamhip_z4c_matter_source and the Z4c boundary.  The lists unew / upre /
  * ucur hold 30 indices: the 25 Z4c fields, then D, tau, S_x, S_y, S_z. */
@@ -557,12 +603,16 @@ typedef struct {
   const int *prim;                 /* 8: primVars (rho, epsl, v^i, p, v2, detg) */
   const int *madm;                 /* 11: matter ADM variables, written */
   int *status, *iters;             /* device arrays of npoints ints for the c2p */
+  int atm_prerhs;                  /* grhd_use_atmosphere_prerhs (reference default yes): after the c2p,
+                                      bamhip_grhd_atm_set_prerhs on upre (matter.c:414-419) */
 } bamhip_z4c_grhd;
 int bamhip_z4c_grhd_rhs(bamhip_level *lev, const bamhip_z4c_grhd *S, const int *unew, const int *upre, double c,
                         const int *ucur);
 /* One RK4 step of the coupled system: evolve_rk_rk4 (src/utility/evolve/
- * rk.c:220-257) over the 30 fields with bamhip_z4c_grhd_rhs per stage, then
- * algcon on the Z4c fields (z4.c:49-71); lists of 30 indices as above. */
+ * rk.c:220-257) over the 30 fields with bamhip_z4c_grhd_rhs(r, up, 0, src)
+ * per stage (u_p is upre, so the atmosphere reset acts on it), then algcon
+ * on the Z4c fields (z4.c:49-71); lists of 30 indices as above.  The level's
+ * check mode is restored on return. */
 int bamhip_z4c_grhd_rk4_step(bamhip_level *lev, const bamhip_z4c_grhd *S, double dt, const int *uc, const int *up,
                              const int *r, const int *v, int algcon);
 
@@ -639,6 +689,8 @@ typedef struct {
   int nvars;
   const int *vars;
   const char *const *var_names;
+  const double *dt;   /* per-level time step or NULL: reading returns time = iteration * dt, as the reference
+                         does (checkpoint_read.c:117-124); NULL: the header's time (printed with %g) */
 } bamhip_checkpoint;
 int bamhip_checkpoint_write(const char *path, const bamhip_checkpoint *C);
 int bamhip_checkpoint_read(const char *path, const bamhip_checkpoint *C, int *iteration, double *time);

@@ -133,6 +133,8 @@ def lib():
         L.ob_c2p_proot.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP, IP, IP]
         L.ob_c2p_bracketed.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP, IP, IP]
         L.ob_p2c.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP]
+        L.ob_grhd_atm_set_u.restype = C.c_long
+        L.ob_grhd_atm_set_u.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP]
         L.ob_try_hot.restype = C.c_int
         L.ob_try_hot.argtypes = [C.POINTER(C2PParams), C.c_double, C.c_double, C.c_double,
                                  DP, DP, DP, C.c_double, IP]
@@ -409,6 +411,19 @@ def c2p(P, cons, metric, pguess, rhoguess=None, method="hybrid"):
     fn(C.byref(P), n, cons.ctypes.data_as(DP), metric.ctypes.data_as(DP), prim.ctypes.data_as(DP),
        st.ctypes.data_as(IP), it.ctypes.data_as(IP))
     return prim, st, it, cons
+
+
+def grhd_atm_set_u(P, cons, prim):
+    """grhd_atm_set_u on u_p: cons[5, n] (D, tau, S_i) and prim[7] (detg)
+    modified in place from prim[0] (rho); returns the count of points left
+    non-finite (errorexit in the reference)"""
+    assert cons.flags.c_contiguous and prim.flags.c_contiguous and cons.dtype == np.float64
+    n = cons.shape[1]
+    rho = np.ascontiguousarray(prim[0], dtype=np.float64)
+    detg = prim[7]
+    assert detg.flags.c_contiguous
+    return int(lib().ob_grhd_atm_set_u(C.byref(P), n, cons.ctypes.data_as(DP), rho.ctypes.data_as(DP),
+                                       detg.ctypes.data_as(DP)))
 
 
 # ---------------------------------------------------------------- punctures

@@ -442,3 +442,37 @@ void ob_c2p_bracketed(const ob_c2p_params *P, long n, const double *cons, const 
     prim[6 * n + q] = vlx * vx + vly * vy + vlz * vz;
   }
 }
+
+/* grhd_atm_set_u (src/projects/grhd/grhd_atm.c:217-262).  The levels
+ * rhoatmlevel = RHOATM*FATM, Datmlevel = RHOATM, Tatmlevel = RHOATM*EPSLATM
+ * are formed first (:234-236); DETGMIN is grhd.h's 1e-5 (grhd_atm.c includes
+ * grhd.h); DMAX(detg, DETGMIN) keeps the macro's comparison, so a NaN detg
+ * becomes DETGMIN. */
+long ob_grhd_atm_set_u(const ob_c2p_params *P, long n, double *cons, const double *rho, double *detg)
+{
+  const double rhoatmlevel = P->rhoatm * P->fatm;
+  const double Datmlevel = P->rhoatm;
+  const double Tatmlevel = P->rhoatm * P->epslatm;
+  double *MD = cons, *MT = cons + n, *MSx = cons + 2 * n, *MSy = cons + 3 * n, *MSz = cons + 4 * n;
+  long bad = 0;
+  for (long q = 0; q < n; q++) {
+    if (rho[q] < rhoatmlevel) {
+      double SQRTdetg;
+      detg[q] = DMAX(detg[q], 1e-5);
+      SQRTdetg = sqrt(detg[q]);
+      if (!isfinite(SQRTdetg)) SQRTdetg = 1.;
+      MD[q] = SQRTdetg * Datmlevel;
+      MSx[q] = 0.;
+      MSy[q] = 0.;
+      MSz[q] = 0.;
+      MT[q] = SQRTdetg * Tatmlevel;
+    }
+    if (!isfinite(MD[q]) || !isfinite(MSx[q]) || !isfinite(MSy[q]) || !isfinite(MSz[q]) || !isfinite(MT[q]) ||
+        !isfinite(detg[q])) {
+      MD[q] = MSx[q] = MSy[q] = MSz[q] = MT[q] = 0.;
+      detg[q] = 1.;
+      bad++;
+    }
+  }
+  return bad;
+}

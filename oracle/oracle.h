@@ -147,6 +147,12 @@ int ob_try_cold(const ob_c2p_params *P, double D, double S2, double *T,
  * p,v2,detg; p and rho hold the previous values on entry), status/iters: n */
 void ob_c2p_hybrid(const ob_c2p_params *P, long n, double *cons, const double *metric,
                    double *prim, int *status, int *iters);
+/* grhd_atm_set_u (grhd_atm.c:217-262), the body of grhd_atm_set_prerhs
+ * (:268-278) on u_p: cons 5*n (D,T,Sx,Sy,Sz) reset to the cold static
+ * atmosphere where rho < fatm*rhoatm, detg[n] floored at DETGMIN = 1e-5
+ * there.  Returns the number of points left non-finite (errorexit in the
+ * reference; zeroed here, detg = 1). */
+long ob_grhd_atm_set_u(const ob_c2p_params *P, long n, double *cons, const double *rho, double *detg);
 /* prim -> cons (grhd_p2c.c:11-86) */
 void ob_p2c(const ob_c2p_params *P, long n, const double *prim5, const double *metric,
             double *cons);

@@ -67,6 +67,11 @@ def test_checkpoint_write_read_roundtrip(tmp_path):
             lev.fill(v, -1.0)
     it, tm = B.checkpoint_read(path, fresh, [1, 0], ["betax", "alpha"])
     assert it == [4, 8] and tm == [0.5, 0.5]
+    # with the level time steps the restart time is iteration * dt (the
+    # header's %g keeps 6 digits): dt = 1/3 / 4 on level 0, half on level 1
+    dt = [1.0 / 12.0, 1.0 / 24.0]
+    it, tm = B.checkpoint_read(path, fresh, [1, 0], ["betax", "alpha"], dt=dt)
+    assert it == [4, 8] and tm == [4 * dt[0], 8 * dt[1]]
     for li, lev in enumerate(fresh):
         np.testing.assert_array_equal(lev.get(0), data[(li, 0)])
         np.testing.assert_array_equal(lev.get(1), data[(li, 1)])

@@ -1,0 +1,278 @@
+"""The cold static atmosphere on the device against the oracle.
+
+* bamhip_grhd_atm_set_prerhs (grhd_atm_set_prerhs -> grhd_atm_set_u on u_p,
+  src/projects/grhd/grhd_atm.c:217-278) on a two-box level: bitwise, with the
+  detg floor and the non-finite rule in both check modes;
+* the coupled Z4c + GRHD right-hand side and one RK4 step on a TOV star
+  (Gamma 2, K 100, rho_c 1.28e-3, rho_atm = 1e-10 rho_c) whose exterior is
+  atmosphere, against the oracle composed in matter_rhs's order
+  (matter.c:373-482): c2p (the hybrid rewrites the conserved variables of
+  ucur at COLD/ATM points), the atmosphere of upre, matter_rhs_prepare,
+  sources, fluxes; evolve_rk_rk4 passes u_p as upre at every stage
+  (rk.c:231-252).  The Z4c RHS and z4_to_adm round differently from the
+  oracle (FMA contraction in the tile kernel, pow), so fields are compared at
+  a small multiple of their own scale (RHS) or change (step)."""
+import numpy as np
+import pytest
+
+import baum_amd as B
+import oracle as O
+from c2p_data import sample_prims
+
+pytestmark = pytest.mark.gpu
+
+RHOC, KPOLY, GAMMA = 1.28e-3, 100.0, 2.0
+RHOATM = 1e-10 * RHOC
+
+
+def _c2p_pair():
+    kw = dict(eos="ideal", gamma=GAMMA, K=KPOLY, rhoatm=RHOATM, errormax=1e-12, countmax=100)
+    return B.c2p_params(**kw), O.c2p_params(**kw)
+
+
+def test_atm_set_prerhs_matches_oracle():
+    P, OP = _c2p_pair()
+    boxes = [B.make_box(12, 10, 9, 0.0, 0.0, 0.0, 0.5), B.make_box(7, 8, 6, 10.0, 0.0, 0.0, 0.5)]
+    n = 12 * 10 * 9 + 7 * 8 * 6
+    prim5, g, _ = sample_prims(n, seed=5)
+    rng = np.random.default_rng(9)
+    thr = OP.fatm * RHOATM
+    rho = thr * 10.0 ** rng.uniform(-3, 3, n)
+    rho[:3] = [thr, np.nextafter(thr, 0), 0.0]
+    cons = O.p2c(OP, prim5, g)
+    prim = np.zeros((8, n))
+    prim[0] = rho
+    prim[5] = 1e-3 * rho
+    prim[7] = 1.0 + 0.1 * rng.standard_normal(n)
+    prim[7, 3:6] = [1e-7, 0.0, np.nan]              # floored at DETGMIN (NaN fails DMAX's comparison)
+    rho[3:6] = 0.0
+    prim[0] = rho
+    want_c, want_p = cons.copy(), prim.copy()
+    assert O.grhd_atm_set_u(OP, want_c, want_p) == 0
+    UP, PRIM = list(range(5)), list(range(5, 13))
+    lev = B.Level(boxes, 13)
+    lev.enable(range(13))
+    lev.put_state(UP, cons)
+    lev.put_state(PRIM, prim)
+    lev.grhd_atm_set_prerhs(P, UP, PRIM)
+    got_c, got_p = lev.get_state(UP), lev.get_state(PRIM)
+    lev.free()
+    atm = rho < thr
+    assert 0.3 < atm.mean() < 0.7
+    np.testing.assert_array_equal(got_c, want_c)
+    np.testing.assert_array_equal(got_p, want_p)
+
+
+def test_atm_set_prerhs_nonfinite_is_an_error():
+    """a non-finite conserved value the atmosphere does not cure: zeroed
+    (detg = 1) and reported, at once in check mode 1 and through the deferred
+    flag in mode 0; a NaN at an atmosphere point is cured"""
+    P, OP = _c2p_pair()
+    lev = B.Level([B.make_box(8, 8, 8, 0.0, 0.0, 0.0, 0.5)], 13)
+    lev.enable(range(13))
+    UP, PRIM = list(range(5)), list(range(5, 13))
+    n = 512
+
+    def load():
+        cons = np.full((5, n), 1e-6)
+        prim = np.zeros((8, n))
+        prim[0] = 1e-6
+        prim[7] = 1.0
+        prim[0, 7] = 0.0                 # atmosphere point with a NaN: cured
+        cons[2, 7] = np.nan
+        cons[3, 100] = np.nan            # matter point: not cured
+        lev.put_state(UP, cons)
+        lev.put_state(PRIM, prim)
+
+    load()
+    assert lev.get_check() == 1
+    with pytest.raises(B.BamhipError, match="nans not cured"):
+        lev.grhd_atm_set_prerhs(P, UP, PRIM)
+    got_c, got_p = lev.get_state(UP), lev.get_state(PRIM)
+    assert np.all(got_c[:, 100] == 0.0) and got_p[7, 100] == 1.0
+    assert np.all(np.isfinite(got_c)) and got_c[0, 7] == RHOATM and got_c[2, 7] == 0.0
+    load()
+    lev.set_check(0)
+    lev.nonfinite(reset=True)
+    lev.grhd_atm_set_prerhs(P, UP, PRIM)
+    assert lev.nonfinite(reset=True) == 1
+    lev.set_check(1)
+    lev.free()
+
+
+# ------------------------------------------------------------- TOV + atmosphere
+N, H = 40, 0.5
+X0 = -(N - 1) / 2 * H
+
+
+def _tov_state():
+    """Z4c fields and densitised conserved variables of the TOV star on an
+    N^3 box (conformally flat, gt = delta, chi = psi^-4, alpha from set_TOV),
+    seeded 1e-4 density noise, a slow rotation inside the star, atmosphere
+    rho_atm (1 + noise) outside"""
+    hc = float(np.log1p(GAMMA * KPOLY / (GAMMA - 1) * RHOC ** (GAMMA - 1)))
+    star = B.tov_h(hc, KPOLY, GAMMA, 20000)
+    ADMG, ADMK, PRIM = list(range(6)), list(range(6, 12)), list(range(13, 19))
+    lev = B.Level([B.make_box(N, N, N, X0, X0, X0, H)], 19)
+    lev.enable(range(19))
+    for v in range(19):
+        lev.fill(v, 0.0)
+    lev.tov_set(star, ADMG, ADMK, 12, PRIM)
+    psi4, alpha, rho_star = lev.get(ADMG[0]), lev.get(12), lev.get(PRIM[0])
+    lev.free()
+    shape = (N, N, N)
+    x = X0 + H * np.arange(N)
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    rng = np.random.default_rng(20261203)
+    rho0 = np.maximum(rho_star, RHOATM)
+    rho = rho0 * (1 + 1e-4 * rng.standard_normal(rho0.shape))
+    eps = KPOLY * rho ** (GAMMA - 1) / (GAMMA - 1)
+    w = (0.02 * rho_star / RHOC).reshape(shape)
+    vel = np.array([-w * Y, w * X, 0.01 * w * Z]).reshape(3, -1) / psi4
+    u = np.zeros((25,) + shape)
+    u[0] = u[3] = u[5] = 1.0
+    u[6] = (1.0 / psi4).reshape(shape)
+    u[18] = alpha.reshape(shape)
+    g = np.zeros((6, N ** 3))
+    g[0] = g[3] = g[5] = psi4
+    _, OP = _c2p_pair()
+    cons = O.p2c(OP, np.array([rho, eps, vel[0], vel[1], vel[2]]), g)
+    return u, cons.reshape((5,) + shape), rho.reshape(shape), (KPOLY * rho ** GAMMA).reshape(shape)
+
+
+def _oracle_rhs(box, oz4, oc2p, ohp, ucur, upre, pguess, rhoguess):
+    """the coupled RHS in evolve_rhs / matter_rhs order; the c2p rewrites the
+    conserved variables of ucur, the atmosphere those of upre (both in place)"""
+    u = np.ascontiguousarray(ucur[:25])
+    rz, nbad = O.z4c_rhs(box, oz4, u)
+    assert nbad == 0
+    O.z4c_radiative_bc(box, oz4, rz, u)
+    gadm, kadm = O.z4c_to_adm(box, u)
+    prim, st, it, cons_after = O.c2p(oc2p, ucur[25:].reshape(5, -1), gadm.reshape(6, -1), pguess.reshape(-1),
+                                     rhoguess=rhoguess.reshape(-1), method="hybrid")
+    ucur[25:] = cons_after.reshape(ucur[25:].shape)
+    if upre is not None:
+        qp = np.ascontiguousarray(upre[25:].reshape(5, -1))
+        assert O.grhd_atm_set_u(oc2p, qp, prim) == 0
+        upre[25:] = qp.reshape(upre[25:].shape)
+    prim = prim.reshape((8,) + u.shape[1:])
+    metric = np.concatenate([u[18:22], gadm])
+    rq = np.zeros((5,) + u.shape[1:])
+    madm = np.zeros((11,) + u.shape[1:])
+    O.grhd_sources(box, ohp, metric, kadm, prim[:6], rq, madm, rhoatm=oc2p.rhoatm)
+    O.grhd_fluxes(box, ohp, metric, prim[:5], rq)
+    O.z4c_matter_source(box, oz4, u, madm, rz)
+    return np.concatenate([rz, rq]), prim, st
+
+
+def _close(got, want, tol, what, ref=None):
+    worst = 0.0
+    for f in range(want.shape[0]):
+        scale = np.abs(want[f] if ref is None else ref[f]).max()
+        err = np.abs(got[f] - want[f]).max()
+        assert err <= tol * max(scale, 1e-300), (what, f, err, scale)
+        worst = max(worst, err / max(scale, 1e-300))
+    print("%s: max error / scale %.2e (bar %.0e)" % (what, worst, tol), flush=True)
+
+
+def test_coupled_rhs_with_atmosphere_matches_oracle():
+    z4, oz4 = B.z4c_params(), O.z4c_params()
+    P, OP = _c2p_pair()
+    hp = B.hrsc_params(eos="ideal", gamma=GAMMA, K=KPOLY)
+    ohp = O.hrsc_params(eos=0, gamma=GAMMA, K=KPOLY)
+    box = O.make_box(N, N, N, X0, X0, X0, H)
+    u, cons, rho, p = _tov_state()
+    u30 = np.concatenate([u, cons])
+    up30 = u30.copy()
+    up30[25:] *= 1 + 1e-3                 # upre differs from ucur, so the reset is visible
+    want_u, want_up = u30.copy(), up30.copy()
+    want_r, want_prim, want_st = _oracle_rhs(box, oz4, OP, ohp, want_u, want_up, p, rho)
+    atm = want_st == O.C2P_ATM
+    assert 0.2 < atm.mean() < 0.9 and np.any(want_st == O.C2P_OK)
+    UC, UP, R = list(range(30)), list(range(30, 60)), list(range(60, 90))
+    ADMG, ADMK, PRIM, MADM = list(range(90, 96)), list(range(96, 102)), list(range(102, 110)), list(range(110, 121))
+    lev = B.Level([B.make_box(N, N, N, X0, X0, X0, H)], 121)
+    lev.enable(range(121))
+    for v in range(121):
+        lev.fill(v, 0.0)
+    lev.put_state(UC, u30.reshape(30, -1))
+    lev.put_state(UP, up30.reshape(30, -1))
+    lev.put(PRIM[0], rho.reshape(-1))
+    lev.put(PRIM[5], p.reshape(-1))
+    status = B.DeviceArray((N ** 3,), np.int32)
+    iters = B.DeviceArray((N ** 3,), np.int32)
+    lev.z4c_grhd_rhs(z4, P, hp, R, UC, ADMG, ADMK, PRIM, MADM, status, iters, upre=UP, method=B.C2P_HYBRID)
+    got_r = lev.get_state(R).reshape(want_r.shape)
+    got_u = lev.get_state(UC).reshape(want_u.shape)
+    got_up = lev.get_state(UP).reshape(want_up.shape)
+    got_prim = lev.get_state(PRIM).reshape(want_prim.shape)
+    got_st = status.get()
+    status.free()
+    iters.free()
+    lev.free()
+    np.testing.assert_array_equal(got_st, want_st)
+    # the atmosphere of u_p: D = sqrt(detg) rho_atm, tau / D = eps_atm, S = 0 at the ATM points
+    rq = (want_prim[0].reshape(-1) < OP.fatm * OP.rhoatm)
+    assert rq.sum() == atm.sum()
+    assert np.all(got_up[27:30].reshape(3, -1)[:, rq] == 0.0)
+    assert np.any(got_up[25].reshape(-1)[rq] != up30[25].reshape(-1)[rq])
+    _close(got_up[25:], want_up[25:], 1e-14, "u_p conserved after the atmosphere", ref=up30[25:])
+    _close(got_u[25:], want_u[25:], 1e-14, "u_c conserved after the c2p rewrite", ref=u30[25:])
+    _close(got_prim, want_prim, 1e-12, "primitives")
+    _close(got_r[:25], want_r[:25], 1e-10, "Z4c rhs")
+    _close(got_r[25:], want_r[25:], 1e-10, "matter rhs")
+
+
+def test_coupled_rk4_step_with_atmosphere_matches_oracle():
+    """evolve_rk_rk4 over the 30 fields: u_p = u_c, four stages r = F(v) with
+    u_p as upre (the atmosphere is reset in u_p, the c2p rewrites the stage
+    state), u_c += w_c r, v = u_p + w_v r; then algcon"""
+    z4, oz4 = B.z4c_params(), O.z4c_params()
+    P, OP = _c2p_pair()
+    hp = B.hrsc_params(eos="ideal", gamma=GAMMA, K=KPOLY)
+    ohp = O.hrsc_params(eos=0, gamma=GAMMA, K=KPOLY)
+    box = O.make_box(N, N, N, X0, X0, X0, H)
+    u, cons, rho, p = _tov_state()
+    u30 = np.concatenate([u, cons])
+    dt = 0.25 * H
+    uc, up = u30.copy(), u30.copy()
+    pg, rg = p.copy(), rho.copy()
+    wc, wv = [dt / 6, dt / 3, dt / 3, dt / 6], [dt / 2, dt / 2, dt, 0.0]
+    v = None
+    for s in range(4):
+        r, prim, st = _oracle_rhs(box, oz4, OP, ohp, uc if s == 0 else v, up, pg, rg)
+        pg, rg = prim[5], prim[0]
+        uc = uc + wc[s] * r
+        v = up + wv[s] * r
+    O.z4c_algcon(box, uc[:25])
+    assert np.any(st == O.C2P_ATM) and np.any(st == O.C2P_OK)
+    UC, UP, R, V = (list(range(30 * k, 30 * k + 30)) for k in range(4))
+    ADMG, ADMK = list(range(120, 126)), list(range(126, 132))
+    PRIM, MADM = list(range(132, 140)), list(range(140, 151))
+    lev = B.Level([B.make_box(N, N, N, X0, X0, X0, H)], 151)
+    lev.enable(range(151))
+    for k in range(151):
+        lev.fill(k, 0.0)
+    lev.put_state(UC, u30.reshape(30, -1))
+    lev.put(PRIM[0], rho.reshape(-1))
+    lev.put(PRIM[5], p.reshape(-1))
+    status = B.DeviceArray((N ** 3,), np.int32)
+    iters = B.DeviceArray((N ** 3,), np.int32)
+    lev.set_check(0)
+    lev.z4c_grhd_rk4_step(z4, P, hp, dt, UC, UP, R, V, ADMG, ADMK, PRIM, MADM, status, iters,
+                          method=B.C2P_HYBRID)
+    assert lev.get_check() == 0                  # the caller's mode is restored
+    got = lev.get_state(UC).reshape(uc.shape)
+    got_up = lev.get_state(UP).reshape(up.shape)
+    got_st = status.get()
+    status.free()
+    iters.free()
+    lev.free()
+    np.testing.assert_array_equal(got_st, st)
+    _close(got_up[25:], up[25:], 1e-13, "u_p conserved (atmosphere of the last stage)", ref=u30[25:])
+    change = uc - u30
+    for f in range(30):
+        if not np.any(change[f]):
+            np.testing.assert_array_equal(got[f], uc[f])
+    _close(got - u30, change, 1e-10, "RK4 step change per field",
+           ref=np.where(np.abs(change).max(axis=(1, 2, 3), keepdims=True) > 0, change, 1.0))

@@ -19,6 +19,9 @@ U, UP, UPP, R, V = (list(range(25 * k, 25 * k + 25)) for k in range(5))
 FP, FR = 125, 126
 NVAR, NB, ORDER, NXYZ = 127, 6, 4, 24
 H0 = 0.5
+# 24 points per direction on the moving levels, without the extra buffer
+# points (the size rules themselves are pinned on the host)
+SIZES = B.NestedSizes(nx=NXYZ, move_nxyz=NXYZ, nbuffer=NB, addextra=False)
 
 
 def coarse_level():
@@ -28,7 +31,7 @@ def coarse_level():
 
 def nested_boxes(points, h):
     """the fine boxes move_box makes on a parent of spacing h"""
-    bb, ib = B.find_nested_bboxes(points, h, NXYZ)
+    bb, ib = B.find_nested_bboxes(SIZES, 1, points, h)
     out = []
     for b, i in zip(bb, ib):
         d = h / 2
@@ -121,7 +124,7 @@ def check_level(old_vals, new, idx, marker_new_region):
 def hierarchy(levels, points, dt0=0.125):
     punc = B.DeviceArray.from_host(np.array(points, dtype=np.float64))
     H = B.Hierarchy(levels, [B.z4c_params()] * len(levels), dt0, U, UP, UPP, R, V, order_RP=ORDER,
-                    timeinterpolation=3, punctures=punc, move_nxyz=NXYZ, rp_flags=(NB, FP, FR))
+                    timeinterpolation=3, punctures=punc, sizes=SIZES, rp_flags=(NB, FP, FR))
     H.set_check(0)
     return H
 
@@ -197,3 +200,20 @@ def test_boxes_follow_punctures_during_evolution():
     assert abs(H.levels[1].boxes[0].x0 - (-2.875)) > 0.1      # level 1 left the origin
     H.advance(0)
     assert H.nonfinite() == 0
+
+
+def test_move_keeps_a_deferred_nonfinite_flag():
+    """a non-finite RHS recorded on a fine level in check mode 0 is still
+    reported after move_box replaced that level"""
+    levels = build([[0.0, 0.0, 0.0]], 2, [0.0, 0.0])
+    H = hierarchy(levels, [[0.9, -0.6, 0.3]])
+    assert H.nonfinite(reset=True) == 0
+    bad = levels[1].get(U[18])
+    bad[bad.size // 2] = np.nan
+    levels[1].put(U[18], bad)
+    levels[1].z4c_rhs(B.z4c_params(), R, U)          # check mode 0: the flag is only recorded
+    old = levels[1]
+    assert H.move_box(1)
+    assert H.levels[1] is not old
+    assert H.nonfinite(reset=True) == 1
+    assert H.nonfinite(reset=True) == 0

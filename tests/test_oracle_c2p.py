@@ -158,3 +158,61 @@ def test_hybrid_cold_fallback_and_atmosphere():
         h = 1 + prim[1, 0] + prim[5, 0] / prim[0, 0]
         Wl = cons_out[0, 0] / prim[0, 0]
         np.testing.assert_allclose(cons_out[1, 0], Wl * Wl * prim[0, 0] * h - prim[5, 0] - cons_out[0, 0], rtol=1e-10)
+
+
+def test_atmosphere_reset_of_upre():
+    """grhd_atm_set_u on u_p (grhd_atm.c:217-262).  Pinned by the static
+    atmosphere itself: the reset conserved state is what prim -> cons
+    (grhd_compute_q_pt, restated independently in ob_p2c) gives for
+    rho = rho_atm, eps = eps_atm, v = 0 on the point's metric, i.e.
+    D = sqrt(detg) rho_atm and tau / D = eps_atm, S_i = 0; points at or above
+    f_atm rho_atm are untouched bitwise; detg is floored at DETGMIN = 1e-5
+    (grhd.h), a non-finite sqrt(detg) becomes 1, and a point left non-finite
+    is zeroed and counted (the reference's errorexit)."""
+    rhoatm = 1.28e-13
+    P = O.c2p_params(eos="ideal", gamma=2.0, K=100.0, rhoatm=rhoatm, errormax=1e-12)
+    prim5, g, _ = sample_prims(4000, seed=11)
+    n = prim5.shape[1]
+    rng = np.random.default_rng(3)
+    thr = P.fatm * rhoatm
+    rho = np.where(rng.uniform(size=n) < 0.5, thr * 10.0 ** rng.uniform(-4, 0, n), thr * 10.0 ** rng.uniform(0, 4, n))
+    rho[:4] = [thr, np.nextafter(thr, 0), 0.0, -1e-30]             # boundary of the test, zero, negative
+    cons = O.p2c(P, prim5, g)
+    prim = np.zeros((8, n))
+    prim[0] = rho
+    c11 = g[3] * g[5] - g[4] ** 2
+    c12 = g[2] * g[4] - g[1] * g[5]
+    c13 = g[1] * g[4] - g[2] * g[3]
+    prim[7] = g[0] * c11 + g[1] * c12 + g[2] * c13
+    prim[7, 4:8] = [1e-7, 0.0, np.inf, np.nan]                    # floored / non-finite detg
+    rho[4:8] = 0.0
+    prim[0] = rho
+    before, detg0 = cons.copy(), prim[7].copy()
+    bad = O.grhd_atm_set_u(P, cons, prim)
+    assert bad == 1                                                   # the inf detg point (below)
+    atm = rho < thr
+    assert atm[1] and not atm[0] and atm[2] and atm[3]
+    np.testing.assert_array_equal(cons[:, ~atm], before[:, ~atm])
+    np.testing.assert_array_equal(prim[7, ~atm], detg0[~atm])
+    # independent prim -> cons of the atmosphere state on the same metrics
+    ga = g[:, atm]
+    pa = np.zeros((5, ga.shape[1]))
+    pa[0], pa[1] = rhoatm, P.epslatm
+    ca = O.p2c(P, pa, ga)
+    finite_det = np.isfinite(detg0[atm]) & (detg0[atm] > 1e-5)
+    np.testing.assert_allclose(cons[0, atm][finite_det], ca[0][finite_det], rtol=2e-16, atol=0)
+    ok = np.arange(n)[atm] != 6
+    np.testing.assert_allclose(cons[1, atm][ok] / cons[0, atm][ok], P.epslatm, rtol=4e-16)
+    np.testing.assert_array_equal(cons[2:, atm], 0.0)
+    # the floor (a NaN detg fails DMAX's comparison and takes the floor too)
+    assert prim[7, 4] == 1e-5 and prim[7, 5] == 1e-5 and prim[7, 7] == 1e-5
+    assert cons[0, 4] == np.sqrt(1e-5) * rhoatm
+    # detg = inf: sqrt(inf) -> 1, but detg itself stays non-finite: zeroed, detg = 1, counted
+    assert prim[7, 6] == 1.0 and np.all(cons[:, 6] == 0.0)
+    # a non-finite conserved value at a point above the threshold is not cured
+    cons2, prim2 = before.copy(), np.zeros((8, n))
+    prim2[0] = 1.0
+    prim2[7] = 1.0
+    cons2[2, 10] = np.nan
+    assert O.grhd_atm_set_u(P, cons2, prim2) == 1
+    assert np.all(cons2[:, 10] == 0.0) and prim2[7, 10] == 1.0
