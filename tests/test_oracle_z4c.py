@@ -396,3 +396,132 @@ def test_periodic_fill():
     for k in range(n):
         kk = (k - 3) % per + 3
         np.testing.assert_array_equal(u[0, k, :, :][3:-3, 3:-3], u[0, kk, :, :][3:-3, 3:-3])
+
+
+def _fd_weights(pos, deriv=1):
+    """weights of the finite-difference approximant of the deriv-th
+    derivative at 0 from the points pos (exact for polynomials of degree
+    < len(pos)): the Vandermonde system sum_k w_k pos_k^p = p! delta_{p,deriv}"""
+    pos = np.asarray(pos, dtype=float)
+    n = len(pos)
+    A = np.array([pos ** p for p in range(n)])
+    b = np.zeros(n)
+    b[deriv] = float(np.prod(np.arange(1, deriv + 1)))
+    return np.linalg.solve(A, b)
+
+
+def test_advection_rows_near_box_edges():
+    """The lop-sided advection row the oracle uses at each point of a line
+    through a box (set_advection_wsign / set_advection4_onedir,
+    bam_amr_advection.h:90-146, with the order-2 rows one point from the
+    edge, bam_amr_loops_box.h:186-215), pinned without the reference's
+    tables: the expected row is the least-shifted upwind stencil (offset at
+    least advection_lopsided = 1 for order 4, 0 for order 2) that stays
+    inside the box [0, imax], with weights derived here from exactness for
+    degree <= 4 (<= 2).  Khat has quintic and sextic parts in x, so every
+    candidate row gives a different value (rows differ in their fifth
+    moments, mirror rows in their sixth); flat space, unit lapse,
+    no dissipation: rhs[Khat] = beta^x d_x Khat + Khat^2 / 3."""
+    m, n = 14, 7
+    h = 0.25
+    p = O.z4c_params(order_dissipation=0)
+    b = O.make_box(m, n, n, -1.3, -0.75, -0.75, h)
+    X = grid(b)
+    imax = m - 1
+    j = k = 3
+    x = X[0][k, j, :]
+    c5 = 1e-3
+
+    def f(xx):
+        t = (xx + 0.37) / 1.7
+        return c5 * (t ** 5 + 0.7 * t ** 6 + 0.3 * ((xx - 0.2) / 1.7) ** 4)
+
+    for bx in (-0.6, 0.45):
+        u = flat_z4c((n, n, m))
+        u[19] = bx
+        u[13] = f(X[0])
+        r, nbad = O.z4c_rhs(b, p, u)
+        assert nbad == 0
+        adv = r[13][k, j, :] - u[13][k, j, :] ** 2 / 3
+        for i in range(1, imax):
+            order = 2 if i in (1, imax - 1) else 4
+            npt = order + 1
+            base = 0 if order == 2 else 1
+            cands = {}
+            for off in range(npt):
+                # upwind: beta < 0 uses points on the low side of i, shifted up by off
+                lo = i + off - order if bx < 0 else i - off
+                pos = np.arange(lo, lo + npt)
+                if pos.min() < 0 or pos.max() > imax:
+                    continue
+                w = _fd_weights(pos - i)
+                cands[off] = bx * np.dot(w, f(x[pos])) / h
+            inside = [o for o in cands if o >= base]
+            want = cands[min(inside)]
+            assert abs(adv[i] - want) <= 1e-12 * abs(want) + 1e-18, (bx, i, adv[i], want)
+            for off, val in cands.items():
+                if off != min(inside):
+                    assert abs(val - want) > 1e-6 * abs(want), (bx, i, off)
+
+
+def test_radiative_bc_neighbour_choice_at_faces_edges_and_corners():
+    """set_boundary_radiative's difference choice (computedel, radiative.c:
+    18-40, neighbours 1 and 2 away that exist in the box, set_nbinds_check_13)
+    at every PHYBOUND point of a box that contains the origin's plane x = 0
+    (so v^x > 0, < 0 and = 0 all occur on the faces, edges and corners),
+    restated by enumerating the existing neighbours of each point: upwind
+    one-sided 2nd-order difference where both upwind neighbours exist, the
+    centred one where only the near one does, else the downwind one-sided;
+    with v^x = 0 the centred one where both near neighbours exist.  Quartic
+    data separate the three differences at every point."""
+    m, nn, o = 9, 6, 7
+    h = 0.5
+    b = O.make_box(m, nn, o, -2.0, 0.3, -1.6, h)
+    X = grid(b)
+    p = O.z4c_params()
+    rng = np.random.default_rng(12)
+    coef = rng.uniform(0.5, 1.5, 9)
+    x, y, z = X
+    Kh = (coef[0] * x ** 4 + coef[1] * y ** 4 + coef[2] * z ** 4 + coef[3] * x ** 3 * y + coef[4] * y * z ** 3
+          + coef[5] * x * y * z + coef[6] * x ** 2 + coef[7] * y + coef[8])
+    u = flat_z4c((o, nn, m))
+    u[13] = 1e-2 * Kh
+    rhs = np.zeros_like(u)
+    O.z4c_radiative_bc(b, p, rhs, u)
+    v = np.sqrt(2.0)                        # 1+log with f = 2: speed sqrt(2) for Khat, far limit 0
+    U = u[13]
+    dims = (m, nn, o)
+    nface = 0
+    for kk in range(o):
+        for jj in range(nn):
+            for ii in range(m):
+                idx = (ii, jj, kk)
+                if not any(t in (0, d - 1) for t, d in zip(idx, dims)):
+                    continue
+                xs = (x[kk, jj, ii], y[kk, jj, ii], z[kk, jj, ii])
+                rr = np.sqrt(sum(c * c for c in xs))
+                out = -v * (U[kk, jj, ii] - 0.0) / rr
+                acc = 0.0
+                for d in range(3):
+                    def at(s):
+                        q = list(idx)
+                        q[d] += s
+                        return U[q[2], q[1], q[0]]
+                    have = {s for s in (-2, -1, 1, 2) if 0 <= idx[d] + s < dims[d]}
+                    cen = (at(1) if 1 in have else at(0)) - (at(-1) if -1 in have else at(0))
+                    rgt = 4 * at(1) - at(2) - 3 * at(0) if {1, 2} <= have else None
+                    lft = 3 * at(0) + at(-2) - 4 * at(-1) if {-1, -2} <= have else None
+                    vd = v * xs[d] / rr
+                    if vd > 0:
+                        dl = lft if -2 in have else (cen if -1 in have else rgt)
+                    elif vd < 0:
+                        dl = rgt if 2 in have else (cen if -1 in have else lft)
+                    else:
+                        dl = cen if (1 in have and -1 in have) else (rgt if -1 not in have else lft)
+                    if lft is not None and rgt is not None:
+                        assert abs(lft - rgt) > 1e-9 and abs(lft - cen) > 1e-9
+                    acc += vd * dl / (2 * h)
+                out -= acc
+                assert abs(rhs[13][kk, jj, ii] - out) <= 1e-13 * max(abs(out), 1e-12), (idx, rhs[13][kk, jj, ii], out)
+                nface += 1
+    assert nface == m * nn * o - (m - 2) * (nn - 2) * (o - 2)
