@@ -199,6 +199,29 @@ def z4c_params(**over):
     return p
 
 
+class DissipationRule(C.Structure):
+    _fields_ = [("dissipation_factor", C.c_double), ("order_dissipation", C.c_int),
+                ("amr_move_lcube", C.c_int), ("physics_z4d", C.c_int),
+                ("factor_level0", C.c_double), ("factor_shells", C.c_double), ("factor_move", C.c_double)]
+
+
+def dissipation_factor(l, dissipation_factor=0.1, order_dissipation=4, amr_move_lcube=0, physics="z4",
+                       dissipation_factor_level0=-1.0, dissipation_factor_shells=-1.0,
+                       dissipation_factor_move=-1.0, shells=False):
+    """get_dissipation_factor(level l) (evolve.c:486-530) from BAM parameter
+    names; the value for z4c_params' dissipation factor per level"""
+    R = DissipationRule(float(dissipation_factor), int(order_dissipation), int(amr_move_lcube),
+                        _getv(physics, "Z4d"), float(dissipation_factor_level0),
+                        float(dissipation_factor_shells), float(dissipation_factor_move))
+    out = C.c_double()
+    L = lib()
+    L.bamhip_dissipation_factor.restype = C.c_int
+    L.bamhip_dissipation_factor.argtypes = [C.POINTER(DissipationRule), C.c_int, C.c_int, DP]
+    check(L.bamhip_dissipation_factor(C.byref(R), int(l), 1 if shells else 0, C.byref(out)),
+          "bamhip_dissipation_factor")
+    return out.value
+
+
 def make_box(m, n, o, x0, y0, z0, dx, dy=None, dz=None, skip=(1,) * 6, bflag=(PHYBOUND,) * 6):
     b = Box()
     b.m, b.n, b.o = m, n, o
@@ -455,6 +478,7 @@ class Level:
         check(L.bamhip_level_reduce(self.h, len(vars), _iarr(vars), OUTPUT_0D.index(kind), out),
               "bamhip_level_reduce")
         return np.array(out[:])
+
 
     def output_0d(self, vars, names, kind, time, outdir):
         """append one 0d line per variable in the reference's format"""

@@ -120,6 +120,22 @@ typedef struct {
 
 #define BAMHIP_Z4C_NVARS 25
 
+/* The parameters get_dissipation_factor reads (src/utility/evolve/
+ * evolve.c:486-530): dissipation_factor, order_dissipation, amr_move_lcube,
+ * whether physics contains "Z4d", and dissipation_factor_level0, _shells,
+ * _move, where -1 means unset (the defaults, bam_main.c:69-72,
+ * bam_amr.c:85-90). */
+typedef struct {
+  double dissipation_factor;
+  int order_dissipation;
+  int amr_move_lcube;
+  int physics_z4d;
+  double factor_level0, factor_shells, factor_move;
+} bamhip_dissipation_rule;
+/* get_dissipation_factor(level) for level index l (shells: the level holds
+ * shell coordinates): the value for bamhip_z4c_params.dissfactor.  Host only. */
+int bamhip_dissipation_factor(const bamhip_dissipation_rule *R, int l, int shells, double *factor);
+
 /* Replaces z4_rhs_movpunc_N(unew, upre, c, ucur) (z4_rhs_movpunc_N.c:23)
  * registered through set_rhs_func_register("geometry", ...) (evolve.c:185,
  * z4.c:245).  unew/upre/ucur: the 25 variable indices of each list in the
