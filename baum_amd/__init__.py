@@ -424,6 +424,23 @@ class Level:
         check(lib().bamhip_level_nonfinite(self.h, 1 if reset else 0, C.byref(f)), "bamhip_level_nonfinite")
         return f.value
 
+    def grhd_c2p(self, c2p, method, cons5, metric6, prim8, status, iters):
+        """MATTER.c2p on the level (status / iters: DeviceArray of npoints int32)"""
+        L = lib()
+        _sig(L, "bamhip_grhd_c2p", C.c_int, [C.c_void_p, C.POINTER(C2PParams), C.c_int, IP, IP, IP, C.c_void_p,
+                                             C.c_void_p])
+        check(L.bamhip_grhd_c2p(self.h, C.byref(c2p), method, _iarr(cons5), _iarr(metric6), _iarr(prim8),
+                                status.ptr, iters.ptr), "bamhip_grhd_c2p")
+
+    def grhd_c2p_learned(self, c2p, mlp, cons5, metric6, prim8, status, iters, fallbacks=None, polish_max=8):
+        """MATTER.c2p with the learned guess (the reference hybrid's control flow)"""
+        L = lib()
+        _sig(L, "bamhip_grhd_c2p_learned", C.c_int, [C.c_void_p, C.POINTER(C2PParams), C.c_void_p, C.c_int, IP, IP,
+                                                     IP, C.c_void_p, C.c_void_p, C.c_void_p])
+        check(L.bamhip_grhd_c2p_learned(self.h, C.byref(c2p), mlp.h, polish_max, _iarr(cons5), _iarr(metric6),
+                                        _iarr(prim8), status.ptr, iters.ptr,
+                                        fallbacks.ptr if fallbacks is not None else None), "bamhip_grhd_c2p_learned")
+
     def grhd_atm_set_prerhs(self, c2p, upre5, prim8):
         """grhd_atm_set_prerhs: the cold static atmosphere on the conserved
         variables upre5 (D, tau, S_i) where prim8's rho is below fatm*rhoatm"""
@@ -744,7 +761,7 @@ class Hierarchy:
 
 # ---------------------------------------------------------------- con2prim
 C2P_COLD, C2P_ATM, C2P_OK, C2P_FAIL, C2P_NR = 0, 1, 2, 3, 4
-C2P_HYBRID, C2P_PLAIN, C2P_BRACKETED, C2P_LEARNED = 0, 1, 2, 3
+C2P_HYBRID, C2P_PLAIN, C2P_BRACKETED, C2P_LEARNED, C2P_VANAL = 0, 1, 2, 3, 4
 
 
 class C2PParams(C.Structure):

@@ -172,80 +172,109 @@ struct C2PArrays {
   int *iters;
 };
 
-// hybrid p-root with cold static atmosphere (grhd_c2p_v2.c:303-460)
+// The hybrid p-root with cold static atmosphere (grhd_c2p_v2.c:303-460) in
+// two halves around its hot solve, shared by the reference's hybrid (hot
+// solve started at the previous pressure) and the learned variant (hot
+// solve started at the network's pressure).
+struct HybPoint {
+  double gxx, gxy, gxz, gyy, gyz, gzz;
+  double detg, SQRTdetg;
+  double conD, conT, conSx, conSy, conSz, conSix, conSiy, conSiz, conS2;
+  int result;  // C2P_OK, or C2P_FAIL for unusable input
+};
+
+// metric inverse, undensitised conserved variables and the input checks
+// (grhd_c2p_v2.c:303-370)
+__device__ __forceinline__ void hybrid_prepare(const C2PArrays &A, long q, HybPoint &h) {
+  h.gxx = A.g[0][q], h.gxy = A.g[1][q], h.gxz = A.g[2][q], h.gyy = A.g[3][q], h.gyz = A.g[4][q], h.gzz = A.g[5][q];
+  double gixx, gixy, gixz, giyy, giyz, gizz;
+  h.detg = invg(h.gxx, h.gxy, h.gxz, h.gyy, h.gyz, h.gzz, &gixx, &gixy, &gixz, &giyy, &giyz, &gizz);
+  h.SQRTdetg = sqrt(fabs(h.detg));
+  if (!isfinite(h.SQRTdetg)) {
+    h.detg = 1.;
+    h.SQRTdetg = 1.;
+  }
+  const double oo = 1. / h.SQRTdetg;
+  h.conD = A.cons[0][q] * oo, h.conT = A.cons[1][q] * oo;
+  h.conSx = A.cons[2][q] * oo, h.conSy = A.cons[3][q] * oo, h.conSz = A.cons[4][q] * oo;
+  h.conSix = gixx * h.conSx + gixy * h.conSy + gixz * h.conSz;
+  h.conSiy = gixy * h.conSx + giyy * h.conSy + giyz * h.conSz;
+  h.conSiz = gixz * h.conSx + giyz * h.conSy + gizz * h.conSz;
+  h.conS2 = h.conSx * h.conSix + h.conSy * h.conSiy + h.conSz * h.conSiz;
+  const double SQRTconS2 = sqrt(fabs(h.conS2));
+  h.result = C2P_OK;
+  if (!isfinite(h.conD) || !isfinite(h.conSx) || !isfinite(h.conSy) || !isfinite(h.conSz) || !isfinite(h.conT))
+    h.result = C2P_FAIL;
+  if (!isfinite(h.gxx) || !isfinite(h.gxy) || !isfinite(h.gxz) || !isfinite(h.gyy) || !isfinite(h.gyz) ||
+      !isfinite(h.gzz) || !isfinite(h.detg) || !isfinite(gixx) || !isfinite(gixy) || !isfinite(gixz) ||
+      !isfinite(giyy) || !isfinite(giyz) || !isfinite(gizz))
+    h.result = C2P_FAIL;
+  if (!isfinite(h.conSix) || !isfinite(h.conSiy) || !isfinite(h.conSiz) || !isfinite(h.conS2) ||
+      !isfinite(SQRTconS2)) {
+    h.conSx = h.conSix = 0.;
+    h.conSy = h.conSiy = 0.;
+    h.conSz = h.conSiz = 0.;
+    h.conS2 = 0.;
+    h.result = C2P_FAIL;
+  }
+}
+
+// after the hot solve: the cold branch (rewrites tau), the atmosphere (any
+// other failure; rewrites every conserved variable) and the primitive stores
+// (grhd_c2p_v2.c:372-460)
+__device__ __forceinline__ void hybrid_finish(const C2PArrays &A, const C2PK &P, long q, HybPoint &h, int result,
+                                              double rho, double epsl, double pres, int it) {
+  int it2 = 0;
+  if (result == C2P_COLD) {
+    result = try_cold(P, h.conD, h.conS2, &h.conT, &rho, &epsl, &pres, A.prim[0][q], &it2);
+    A.cons[1][q] = h.SQRTdetg * h.conT;
+  }
+  const int status = result;
+  if (result != C2P_OK && result != C2P_ATM) result = C2P_ATM;
+  if (result == C2P_ATM) {
+    pres = P.patm;
+    rho = P.rhoatm;
+    epsl = P.epslatm;
+    h.conD = P.rhoatm;
+    h.conSx = h.conSix = 0.;
+    h.conSy = h.conSiy = 0.;
+    h.conSz = h.conSiz = 0.;
+    h.conS2 = 0.;
+    h.conT = P.rhoatm * P.epslatm;
+    A.cons[0][q] = h.SQRTdetg * h.conD;
+    A.cons[2][q] = h.SQRTdetg * h.conSx;
+    A.cons[3][q] = h.SQRTdetg * h.conSy;
+    A.cons[4][q] = h.SQRTdetg * h.conSz;
+    A.cons[1][q] = h.SQRTdetg * h.conT;
+  }
+  const double ooDpT = 1.0 / (h.conD + pres + h.conT);
+  const double vx = h.conSix * ooDpT, vy = h.conSiy * ooDpT, vz = h.conSiz * ooDpT;
+  A.prim[0][q] = rho;
+  A.prim[1][q] = epsl;
+  A.prim[2][q] = vx;
+  A.prim[3][q] = vy;
+  A.prim[4][q] = vz;
+  A.prim[5][q] = pres;
+  const double vlx = h.gxx * vx + h.gxy * vy + h.gxz * vz;
+  const double vly = h.gxy * vx + h.gyy * vy + h.gyz * vz;
+  const double vlz = h.gxz * vx + h.gyz * vy + h.gzz * vz;
+  A.prim[6][q] = vlx * vx + vly * vy + vlz * vz;
+  A.prim[7][q] = h.detg;
+  A.status[q] = status;
+  A.iters[q] = it + it2;
+}
+
+// hybrid p-root with cold static atmosphere (grhd_c2p_v2.c:265-464): the hot
+// solve starts at the previous pressure
 __global__ void c2p_hybrid_kernel(C2PArrays A, C2PK P, long n) {
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    const double gxx = A.g[0][q], gxy = A.g[1][q], gxz = A.g[2][q], gyy = A.g[3][q], gyz = A.g[4][q],
-                 gzz = A.g[5][q];
-    double gixx, gixy, gixz, giyy, giyz, gizz;
-    double detg = invg(gxx, gxy, gxz, gyy, gyz, gzz, &gixx, &gixy, &gixz, &giyy, &giyz, &gizz);
-    double SQRTdetg = sqrt(fabs(detg));
-    if (!isfinite(SQRTdetg)) {
-      detg = 1.;
-      SQRTdetg = 1.;
-    }
-    const double oo = 1. / SQRTdetg;
-    double conD = A.cons[0][q] * oo, conT = A.cons[1][q] * oo;
-    double conSx = A.cons[2][q] * oo, conSy = A.cons[3][q] * oo, conSz = A.cons[4][q] * oo;
-    double conSix = gixx * conSx + gixy * conSy + gixz * conSz;
-    double conSiy = gixy * conSx + giyy * conSy + giyz * conSz;
-    double conSiz = gixz * conSx + giyz * conSy + gizz * conSz;
-    double conS2 = conSx * conSix + conSy * conSiy + conSz * conSiz;
-    double SQRTconS2 = sqrt(fabs(conS2));
-    int result = C2P_OK;
-    if (!isfinite(conD) || !isfinite(conSx) || !isfinite(conSy) || !isfinite(conSz) || !isfinite(conT))
-      result = C2P_FAIL;
-    if (!isfinite(gxx) || !isfinite(gxy) || !isfinite(gxz) || !isfinite(gyy) || !isfinite(gyz) || !isfinite(gzz) ||
-        !isfinite(detg) || !isfinite(gixx) || !isfinite(gixy) || !isfinite(gixz) || !isfinite(giyy) ||
-        !isfinite(giyz) || !isfinite(gizz))
-      result = C2P_FAIL;
-    if (!isfinite(conSix) || !isfinite(conSiy) || !isfinite(conSiz) || !isfinite(conS2) || !isfinite(SQRTconS2)) {
-      conSx = conSix = 0.;
-      conSy = conSiy = 0.;
-      conSz = conSiz = 0.;
-      conS2 = SQRTconS2 = 0.;
-      result = C2P_FAIL;
-    }
+    HybPoint h;
+    hybrid_prepare(A, q, h);
     double rho = 0, epsl = 0, pres = 0;
-    int it = 0, it2 = 0;
-    if (result != C2P_FAIL) result = try_hot(P, conD, conS2, conT, &rho, &epsl, &pres, A.prim[5][q], &it);
-    if (result == C2P_COLD) {
-      result = try_cold(P, conD, conS2, &conT, &rho, &epsl, &pres, A.prim[0][q], &it2);
-      A.cons[1][q] = SQRTdetg * conT;
-    }
-    int status = result;
-    if (result != C2P_OK && result != C2P_ATM) result = C2P_ATM;
-    if (result == C2P_ATM) {
-      pres = P.patm;
-      rho = P.rhoatm;
-      epsl = P.epslatm;
-      conD = P.rhoatm;
-      conSx = conSix = 0.;
-      conSy = conSiy = 0.;
-      conSz = conSiz = 0.;
-      conS2 = 0.;
-      conT = P.rhoatm * P.epslatm;
-      A.cons[0][q] = SQRTdetg * conD;
-      A.cons[2][q] = SQRTdetg * conSx;
-      A.cons[3][q] = SQRTdetg * conSy;
-      A.cons[4][q] = SQRTdetg * conSz;
-      A.cons[1][q] = SQRTdetg * conT;
-    }
-    const double ooDpT = 1.0 / (conD + pres + conT);
-    const double vx = conSix * ooDpT, vy = conSiy * ooDpT, vz = conSiz * ooDpT;
-    A.prim[0][q] = rho;
-    A.prim[1][q] = epsl;
-    A.prim[2][q] = vx;
-    A.prim[3][q] = vy;
-    A.prim[4][q] = vz;
-    A.prim[5][q] = pres;
-    const double vlx = gxx * vx + gxy * vy + gxz * vz;
-    const double vly = gxy * vx + gyy * vy + gyz * vz;
-    const double vlz = gxz * vx + gyz * vy + gzz * vz;
-    A.prim[6][q] = vlx * vx + vly * vy + vlz * vz;
-    A.prim[7][q] = detg;
-    A.status[q] = status;
-    A.iters[q] = it + it2;
+    int it = 0;
+    int result = h.result;
+    if (result != C2P_FAIL) result = try_hot(P, h.conD, h.conS2, h.conT, &rho, &epsl, &pres, A.prim[5][q], &it);
+    hybrid_finish(A, P, q, h, result, rho, epsl, pres, it);
   }
 }
 
@@ -491,6 +520,71 @@ __global__ void __launch_bounds__(64) c2p_learned_kernel(C2PArrays A, C2PK P, Ml
   if (fallbacks && l == 0 && nfb) atomicAdd(fallbacks, nfb);
 }
 
+// The hybrid of grhd_c2p_v2.c:265-464 with the learned guess as the hot
+// solve's initial pressure (MATTER.c2p with the network): each wavefront
+// evaluates the network for its 64 points on the matrix cores, then every
+// lane runs the reference's hot solve (try_hot, at most polish_max
+// iterations) from the predicted pressure.  If that does not return OK the
+// point falls back to the reference's hybrid from the previous pressure
+// (counted in *fallbacks), so the cold branch and the atmosphere follow the
+// reference's rules.  A state with D < fatm rhoatm (rho = D / W is then
+// below the atmosphere threshold whatever the pressure) and a state with no
+// thermal content skip the network.  The loop trip count is uniform over
+// the block, so every wave reaches the barriers.
+__global__ void __launch_bounds__(64) c2p_learned_hybrid_kernel(C2PArrays A, C2PK P, MlpView M, long n,
+                                                                int polish_max, int *fallbacks) {
+  __shared__ __attribute__((aligned(16))) __bf16 h1[64 * kHStr];
+  __shared__ __attribute__((aligned(16))) float b2[kMlpH], w3[kMlpH], w1b1[4 * kMlpH];
+  const int l = threadIdx.x;
+  b2[l] = M.p[kOffB2 + l];
+  w3[l] = M.wq[192 + l];
+  for (int j = 0; j < 3; j++) w1b1[3 * l + j] = M.wq[3 * l + j];
+  w1b1[192 + l] = M.p[kOffB1 + l];
+  __syncthreads();
+  C2PK Pl = P;
+  Pl.countmax = polish_max;
+  int nfb = 0;
+  for (long base = (long)blockIdx.x * 64; base < n; base += (long)gridDim.x * 64) {
+    const long q = base + l;
+    const bool act = q < n;
+    HybPoint h;
+    double th = 0.;
+    float x[3] = {0.f, 0.f, 0.f};
+    bool valid = false;
+    if (act) {
+      hybrid_prepare(A, q, h);
+      if (h.result != C2P_FAIL && h.conD >= P.fatm * P.rhoatm) {
+        th = mlp_thermal(h.conD, h.conS2, h.conT);
+        valid = mlp_features(M, h.conD, h.conS2, h.conT, th, x);
+      }
+      if (!valid) x[0] = x[1] = x[2] = 0.f;
+    }
+    mlp_layer1_from(w1b1, x, h1 + l * kHStr);
+    __syncthreads();
+    const float yn = mlp_infer23(M, h1, b2, w3);
+    __syncthreads();
+    if (act) {
+      double rho = 0., epsl = 0., pres = 0.;
+      int it = 0, result = h.result;
+      if (result != C2P_FAIL) {
+        if (valid) {
+          const double guess = (P.gamma - 1.) * th * (double)__expf(M.ys + M.yk * yn);
+          result = try_hot(Pl, h.conD, h.conS2, h.conT, &rho, &epsl, &pres, guess, &it);
+        }
+        if (!valid || result != C2P_OK) {
+          int it2 = 0;
+          if (valid) nfb++;
+          result = try_hot(P, h.conD, h.conS2, h.conT, &rho, &epsl, &pres, A.prim[5][q], &it2);
+          it += it2;
+        }
+      }
+      hybrid_finish(A, P, q, h, result, rho, epsl, pres, it);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) nfb += __shfl_xor(nfb, o);
+  if (fallbacks && l == 0 && nfb) atomicAdd(fallbacks, nfb);
+}
+
 // training data of the learned guess from solved points: raw features
 // (log D, |S|/(tau+D), log(Delta/(tau+D))) and target
 // log(p / ((Gamma-1) Delta)); the target is NaN where the solve did not
@@ -572,6 +666,85 @@ __global__ void atm_set_u_kernel(C2PArrays A, C2PK P, long n, int *flag) {
   }
 }
 
+// analytic Gamma-law recovery (grhd_c2p_vanal, grhd_c2p.c:239-379): |v| is
+// the physical root of v^4 + a3 v^3 + a2 v^2 + a1 v + a0, taken through the
+// largest root of the resolvent cubic and the quadratic factor holding it;
+// the reference's operation order throughout (fmax / fmin clamp, so a NaN
+// root becomes 0).  Stops of the reference become status FAIL.
+__global__ void c2p_vanal_kernel(C2PArrays A, C2PK P, long n) {
+  const double gammamo = P.gamma - 1.;
+  const double oo3 = 1. / 3.;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    double gixx, gixy, gixz, giyy, giyz, gizz;
+    const double detg = invg(A.g[0][q], A.g[1][q], A.g[2][q], A.g[3][q], A.g[4][q], A.g[5][q], &gixx, &gixy,
+                             &gixz, &giyy, &giyz, &gizz);
+    A.prim[7][q] = detg;
+    A.iters[q] = 0;
+    if (detg <= 0. || !isfinite(detg)) {
+      A.status[q] = C2P_FAIL;
+      continue;
+    }
+    const double oo = 1. / sqrt(detg);
+    const double conD = A.cons[0][q] * oo, conT = A.cons[1][q] * oo;
+    const double conSx = A.cons[2][q] * oo, conSy = A.cons[3][q] * oo, conSz = A.cons[4][q] * oo;
+    const double conSix = gixx * conSx + gixy * conSy + gixz * conSz;
+    const double conSiy = gixy * conSx + giyy * conSy + giyz * conSz;
+    const double conSiz = gixz * conSx + giyz * conSy + gizz * conSz;
+    const double conS2 = conSx * conSix + conSy * conSiy + conSz * conSiz;
+    const double S = sqrt(conS2);
+    if (!isfinite(conD) || !isfinite(conSx) || !isfinite(conSy) || !isfinite(conSz) || !isfinite(conT) ||
+        !isfinite(S)) {
+      A.status[q] = C2P_FAIL;
+      continue;
+    }
+    const double D = conD, E = conT + conD;
+    double v = 0.0;
+    if (fabs(S) > 1e-20) {
+      const double SE = S * E, D2 = D * D;
+      const double G = gammamo + 1, Gmosq = gammamo * gammamo;
+      const double div = 1.0 / (Gmosq * (conS2 + D2));
+      const double a3 = (-2.0 * G * gammamo * SE) * div;
+      const double a2 = (G * G * E * E + 2.0 * gammamo * conS2 - Gmosq * D2) * div;
+      const double a1 = (-2.0 * G * SE) * div;
+      const double a0 = conS2 * div;
+      const double i1 = -a2;
+      const double i2 = a3 * a1 - 4.0 * a0;
+      const double i3 = 4.0 * a2 * a0 - a1 * a1 - a3 * a3 * a0;
+      const double iR = (9.0 * i1 * i2 - 27.0 * i3 - 2.0 * i1 * i1 * i1) / 54.0;
+      const double iS = (3.0 * i2 - a2 * a2) / 9.0;
+      const double iT = iR * iR + iS * iS * iS;
+      double ix1;
+      if (iT < 0.)
+        ix1 = 2.0 * pow(sqrt(iR * iR + iT), oo3) * cos(atan2(sqrt(-iT), iR) * oo3) - i1 * oo3;
+      else
+        ix1 = pow(iR + sqrt(iT), oo3) + pow(iR - sqrt(iT), oo3) - i1 * oo3;
+      const double iB = 0.5 * (a3 + sqrt(a3 * a3 - 4.0 * a2 + 4.0 * ix1));
+      const double iC = 0.5 * (ix1 - sqrt(ix1 * ix1 - 4.0 * a0));
+      v = (-iB + sqrt(iB * iB - 4.0 * iC)) * 0.5;
+    }
+    v = fmax(v, 0.0);
+    v = fmin(v, 1.0 - 1.0e-15);
+    double div = v / S;
+    if (fabs(S) < 1e-20) div = 0.0;
+    const double rho = sqrt(1.0 - v * v) * D;
+    const double vx = conSx * div, vy = conSy * div, vz = conSz * div;
+    const double p = gammamo * ((E - conSx * vx - conSy * vy - conSz * vz) - rho);
+    const double epsl = p / (gammamo * rho);
+    if (epsl < 0.) {
+      A.status[q] = C2P_FAIL;
+      continue;
+    }
+    div = 1. / (conD + p + conT);
+    A.prim[0][q] = rho;
+    A.prim[1][q] = epsl;
+    A.prim[5][q] = p;
+    A.prim[2][q] = conSix * div;
+    A.prim[3][q] = conSiy * div;
+    A.prim[4][q] = conSiz * div;
+    A.status[q] = C2P_OK;
+  }
+}
+
 static C2PK make_c2pk(const bamhip_c2p_params *p) {
   C2PK K;
   K.eos = p->eos;
@@ -603,6 +776,10 @@ int c2p_launch(const bamhip_c2p_params *par, int method, long n, C2PArrays &A) {
     c2p_plain_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(A, K, n);
   else if (method == BAMHIP_C2P_BRACKETED)
     c2p_bracketed_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(A, K, n);
+  else if (method == BAMHIP_C2P_VANAL) {
+    if (par->eos != 0) return set_error(BAMHIP_EINVAL, "bamhip_c2p: the analytic recovery needs the ideal gas");
+    c2p_vanal_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(A, K, n);
+  }
   else
     return set_error(BAMHIP_EINVAL, "bamhip_c2p: unknown method %d", method);
   BH_HIP(hipGetLastError());
@@ -634,8 +811,10 @@ static void c2p_flat(C2PArrays &A, long n, double *cons, const double *metric, d
   A.iters = iters;
 }
 
+// hybrid: the reference hybrid's control flow (MATTER.c2p on a level);
+// otherwise the bracketed polish and fallback of the flat-array solver
 static int learned_launch(const bamhip_c2p_params *par, const bamhip_mlp *m, int polish_max, long n, C2PArrays &A,
-                          int *fallbacks) {
+                          int *fallbacks, bool hybrid) {
   if (!par || !m || n < 0 || polish_max < 1) return set_error(BAMHIP_EINVAL, "bamhip_c2p_learned: bad arguments");
   if (par->eos != 0) return set_error(BAMHIP_EINVAL, "bamhip_c2p_learned: the learned guess is trained for the ideal gas");
   if (par->countmax < 1) return set_error(BAMHIP_EINVAL, "bamhip_c2p_learned: countmax < 1");
@@ -643,8 +822,12 @@ static int learned_launch(const bamhip_c2p_params *par, const bamhip_mlp *m, int
   long blocks = (n + 63) / 64;
   if (blocks > 32768) blocks = 32768;
   TimerScope t("c2p");
-  c2p_learned_kernel<<<(unsigned)blocks, 64, 0, stream()>>>(A, make_c2pk(par), mlp_view(m), n, polish_max,
-                                                            fallbacks);
+  if (hybrid)
+    c2p_learned_hybrid_kernel<<<(unsigned)blocks, 64, 0, stream()>>>(A, make_c2pk(par), mlp_view(m), n, polish_max,
+                                                                     fallbacks);
+  else
+    c2p_learned_kernel<<<(unsigned)blocks, 64, 0, stream()>>>(A, make_c2pk(par), mlp_view(m), n, polish_max,
+                                                              fallbacks);
   BH_HIP(hipGetLastError());
   return BAMHIP_OK;
 }
@@ -656,7 +839,7 @@ extern "C" int bamhip_c2p_learned_batch(const bamhip_c2p_params *par, const bamh
     return set_error(BAMHIP_EINVAL, "bamhip_c2p_learned_batch: NULL array");
   C2PArrays A;
   c2p_flat(A, n, cons, metric, prim, status, iters);
-  return learned_launch(par, m, polish_max, n, A, fallbacks);
+  return learned_launch(par, m, polish_max, n, A, fallbacks, false);
 }
 
 extern "C" int bamhip_grhd_c2p_learned(bamhip_level *lev, const bamhip_c2p_params *par, const bamhip_mlp *m,
@@ -675,7 +858,7 @@ extern "C" int bamhip_grhd_c2p_learned(bamhip_level *lev, const bamhip_c2p_param
   for (int k = 0; k < 8; k++) A.prim[k] = tmp[k];
   A.status = status;
   A.iters = iters;
-  return learned_launch(par, m, polish_max, lev->npoints, A, fallbacks);
+  return learned_launch(par, m, polish_max, lev->npoints, A, fallbacks, true);
 }
 
 extern "C" int bamhip_mlp_prepare(const bamhip_c2p_params *par, long n, const double *cons, const double *metric,

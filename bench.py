@@ -9,7 +9,9 @@ Workloads (synthetic data of the configurations' shapes, FP64 throughout):
   48..96 per direction (clipped so that every box nests in its parent with
   a 4-cell margin and the two boxes of a level do not overlap).  Precollapsed
   lapse, zero shift; 1+log / Gamma driver, 4th-order FD with lop-sided
-  advection, KO 0.1, chi floor 1e-5 in the undifferentiated 1/chi terms, dt_l = h_l / 4 with subcycling, time interpolation of
+  advection, KO 0.1 on level 0 and 0.02 on levels 1..6 (the reference's
+  get_dissipation_factor with amr_move_lcube 0), chi floor 1e-5 in the
+  undifferentiated 1/chi terms, dt_l = h_l / 4 with subcycling, time interpolation of
   order 3, amr_nbuffer 6, order_RP 4.  A step is one coarse step (level 0
   once, level l 2^l times, each followed by restriction / prolongation with
   its parent), captured once as a hipGraph and replayed.  Unit: interior
@@ -150,13 +152,18 @@ def run_fmr(args):
         B.set_rp_flags(levels[l - 1], levels[l], parents[l], 6, FP, FR)
     # chi floor of the reference's binary-puncture parameter files (z4_chi_div_floor = 1e-5):
     # without it the 1/chi terms overflow at the punctures on levels >= 4
-    par = B.z4c_params(z4_chi_div_floor=1e-5)
+    # KO 0.1 with the reference's level rule (get_dissipation_factor with
+    # amr_move_lcube = 0 and dissipation_factor_move unset: 0.2 x on levels >= 1)
+    pars = []
+    for l in range(len(levels)):
+        p = B.z4c_params(z4_chi_div_floor=1e-5)
+        p.dissfactor = B.dissipation_factor(l, 0.1)
+        pars.append(p)
     dt0 = 0.25 * boxes[0][0].dx
     # the punctures are tracked on the finest level after each of its steps (analyze_level)
     punc0 = np.array([c for _, c in BINARY], dtype=np.float64)
     punc = B.DeviceArray.from_host(punc0)
-    H = B.Hierarchy(levels, [par] * len(levels), dt0, U, UP, UPP, R, V, order_RP=4, timeinterpolation=3,
-                    punctures=punc)
+    H = B.Hierarchy(levels, pars, dt0, U, UP, UPP, R, V, order_RP=4, timeinterpolation=3, punctures=punc)
     H.set_check(0)
     updates = sum(2 ** l * inner_points(bl) for l, bl in enumerate(boxes))
     # warm-up: the first coarse step eagerly (reduced-order time interpolation), then capture the next
@@ -601,7 +608,9 @@ def main():
         out.update(value=fmr["value"], unit="grid-point-updates/s", ms_per_step=fmr["ms_per_step"])
         out["config"] = {"workload": "Z4c BO evolution of a 7-level FMR hierarchy: 128^3 level 0 + 2 boxes of "
                                      "48..96^3 per level 1..6 around a BL binary (m=0.5 at x=+-3), nbuffer 6, "
-                                     "order_RP 4, time interpolation 3, z4_chi_div_floor 1e-5, KO 0.1, puncture tracking on level 6, "
+                                     "order_RP 4, time interpolation 3, z4_chi_div_floor 1e-5, dissipation_factor 0.1 with "
+                                     "get_dissipation_factor's rule (amr_move_lcube 0, dissipation_factor_move "
+                                     "unset: 0.1 on level 0, 0.02 on levels 1..6), puncture tracking on level 6, "
                                      "one hipGraph per coarse step",
                          "updates_per_coarse_step": fmr["updates_per_step"], "boxes": fmr["boxes"],
                          "parallelism": "single GPU"}

@@ -248,6 +248,14 @@ typedef struct {
  * the rounding floor of the residual; no reference counterpart), which
  * converges where the reference's relative-change test cannot be met */
 enum { BAMHIP_C2P_HYBRID = 0, BAMHIP_C2P_PLAIN = 1, BAMHIP_C2P_BRACKETED = 2 };
+/* the analytic Gamma-law recovery grhd_c2p_vanal (grhd_c2p.c:239-379,
+ * selected by grhd_C2P = v_anal, grhd.c:160): |v| from the closed-form
+ * root of a quartic (resolvent cubic, then a quadratic factor), clamped to
+ * [0, 1 - 1e-15], then rho, p, eps in closed form.  Ideal gas only.  The
+ * reference's stops (detg <= 0, non-finite input, eps < 0) give status
+ * FAIL with the outputs other than detg untouched; v^2 is not written;
+ * iters = 0.  Method value 4 (3 is BAMHIP_C2P_LEARNED). */
+#define BAMHIP_C2P_VANAL 4
 /* status per point: the reference's codes (grhd_c2p_v2.c:14-20) */
 enum { BAMHIP_C2P_COLD = 0, BAMHIP_C2P_ATM = 1, BAMHIP_C2P_OK = 2, BAMHIP_C2P_FAIL = 3, BAMHIP_C2P_NR = 4 };
 
@@ -328,7 +336,15 @@ int bamhip_mlp_predict(const bamhip_mlp *m, long n, const float *feat, float *y)
 int bamhip_c2p_learned_batch(const bamhip_c2p_params *par, const bamhip_mlp *m, int polish_max, long n,
                              double *cons, const double *metric, double *prim, int *status, int *iters,
                              int *fallbacks);
-/* the same on the variables of a level (as bamhip_grhd_c2p) */
+/* MATTER.c2p with the learned guess on the variables of a level (arrays as
+ * bamhip_grhd_c2p): the reference hybrid grhd_c2p_proot_ColdStaticAtm_hybrid
+ * (grhd_c2p_v2.c:265-464) whose hot solve (try_hot, the reference's Newton
+ * iteration and stopping rule) starts at the network's pressure and runs at
+ * most polish_max iterations.  A point where that does not return OK runs
+ * the reference hybrid from the previous pressure prim8[5] (counted in
+ * *fallbacks), so the cold branch, the atmosphere and the conserved
+ * rewrites are the reference's.  States with D < fatm rhoatm (atmosphere
+ * whatever the pressure) or no thermal content skip the network. */
 int bamhip_grhd_c2p_learned(bamhip_level *lev, const bamhip_c2p_params *par, const bamhip_mlp *m, int polish_max,
                             const int *cons5, const int *metric6, const int *prim8, int *status, int *iters,
                             int *fallbacks);

@@ -132,6 +132,7 @@ def lib():
         L.ob_c2p_hybrid.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP, IP, IP]
         L.ob_c2p_proot.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP, IP, IP]
         L.ob_c2p_bracketed.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP, IP, IP]
+        L.ob_c2p_vanal.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP, IP, DP]
         L.ob_p2c.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP]
         L.ob_grhd_atm_set_u.restype = C.c_long
         L.ob_grhd_atm_set_u.argtypes = [C.POINTER(C2PParams), C.c_long, DP, DP, DP]
@@ -411,6 +412,21 @@ def c2p(P, cons, metric, pguess, rhoguess=None, method="hybrid"):
     fn(C.byref(P), n, cons.ctypes.data_as(DP), metric.ctypes.data_as(DP), prim.ctypes.data_as(DP),
        st.ctypes.data_as(IP), it.ctypes.data_as(IP))
     return prim, st, it, cons
+
+
+def c2p_vanal(P, cons, metric, prim0=None):
+    """grhd_c2p_vanal: returns (prim[8,n], status[n], coef[4,n] = a3, a2, a1, a0
+    of the quartic in |v|); prim0 (optional) is the primitive array before
+    the call (v^2 at prim[6] is left as it was)"""
+    cons = np.ascontiguousarray(cons, dtype=np.float64)
+    metric = np.ascontiguousarray(metric, dtype=np.float64)
+    n = cons.shape[1]
+    prim = np.zeros((8, n)) if prim0 is None else np.array(prim0, dtype=np.float64, order="C", copy=True)
+    st = np.zeros(n, dtype=np.int32)
+    coef = np.zeros((4, n))
+    lib().ob_c2p_vanal(C.byref(P), n, cons.ctypes.data_as(DP), metric.ctypes.data_as(DP), prim.ctypes.data_as(DP),
+                       st.ctypes.data_as(IP), coef.ctypes.data_as(DP))
+    return prim, st, coef
 
 
 def grhd_atm_set_u(P, cons, prim):

@@ -5,7 +5,8 @@
  * try_cold_part), :265-464 (grhd_c2p_proot_ColdStaticAtm_hybrid point loop),
  * src/projects/grhd/grhd_p2c.c:11-86 (prim -> cons), the analytic EOS of
  * src/projects/eos/eos_analytic.c:12-72,423-434 and invg of
- * src/projects/adm/adm_utility.c:27-46.
+ * src/projects/adm/adm_utility.c:27-46, and grhd_c2p_vanal of
+ * src/projects/grhd/grhd_c2p.c:239-379.
  */
 #include <math.h>
 #include "oracle.h"
@@ -475,4 +476,101 @@ long ob_grhd_atm_set_u(const ob_c2p_params *P, long n, double *cons, const doubl
     }
   }
   return bad;
+}
+
+/* Analytic Gamma-law recovery grhd_c2p_vanal (grhd_c2p.c:239-379): the
+ * velocity magnitude v = |v| is the root of the quartic
+ *   v^4 + a3 v^3 + a2 v^2 + a1 v + a0 = 0
+ * taken through the resolvent cubic (Ferrari), clamped to [0, 1 - 1e-15]
+ * (fmax/fmin, so a NaN root becomes 0), then rho = sqrt(1 - v^2) D,
+ * p = (G-1) (E - S_i S_i v/|S| - rho), eps = p / ((G-1) rho).  Ideal gas
+ * only (eos_Gamma).  The reference's errorexit cases (detg <= 0 or not
+ * finite, non-finite conserved variables, eps < 0) become status FAIL;
+ * v^2 (prim[6]) is not written, as in the reference.  coef (optional,
+ * 4*n): a3, a2, a1, a0 per point (0 where |S| <= 1e-20). */
+void ob_c2p_vanal(const ob_c2p_params *P, long n, const double *cons, const double *metric,
+                  double *prim, int *status, double *coef)
+{
+  const double gammamo = P->gamma - 1.;
+  const double oo3 = 1. / 3.;
+#pragma omp parallel for schedule(static)
+  for (long q = 0; q < n; q++) {
+    const double *gm = metric;
+    double gixx, gixy, gixz, giyy, giyz, gizz;
+    const double detg = ob_invg(gm[q], gm[n + q], gm[2 * n + q], gm[3 * n + q], gm[4 * n + q], gm[5 * n + q],
+                                &gixx, &gixy, &gixz, &giyy, &giyz, &gizz);
+    prim[7 * n + q] = detg;
+    status[q] = OB_C2P_OK;
+    if (coef) coef[q] = coef[n + q] = coef[2 * n + q] = coef[3 * n + q] = 0.;
+    if (detg <= 0. || !isfinite(detg)) {
+      status[q] = OB_C2P_FAIL;
+      continue;
+    }
+    const double SQRTdetg = sqrt(detg);
+    const double oo = 1. / SQRTdetg;
+    const double conD = cons[q] * oo, conT = cons[n + q] * oo;
+    const double conSx = cons[2 * n + q] * oo, conSy = cons[3 * n + q] * oo, conSz = cons[4 * n + q] * oo;
+    const double conSix = gixx * conSx + gixy * conSy + gixz * conSz;
+    const double conSiy = gixy * conSx + giyy * conSy + giyz * conSz;
+    const double conSiz = gixz * conSx + giyz * conSy + gizz * conSz;
+    const double conS2 = conSx * conSix + conSy * conSiy + conSz * conSiz;
+    const double SQRTconS2 = sqrt(conS2);
+    if (!isfinite(conD) || !isfinite(conSx) || !isfinite(conSy) || !isfinite(conSz) || !isfinite(conT) ||
+        !isfinite(SQRTconS2)) {
+      status[q] = OB_C2P_FAIL;
+      continue;
+    }
+    const double D = conD, S2 = conS2, S = SQRTconS2, E = conT + conD;
+    double v = 0.0;
+    if (fabs(S) > 1e-20) {
+      const double SE = S * E, D2 = D * D;
+      const double Gamma = gammamo + 1, Gammamosq = gammamo * gammamo;
+      const double div = 1.0 / (Gammamosq * (S2 + D2));
+      const double a3 = (-2.0 * Gamma * gammamo * SE) * div;
+      const double a2 = (Gamma * Gamma * E * E + 2.0 * gammamo * S2 - Gammamosq * D2) * div;
+      const double a1 = (-2.0 * Gamma * SE) * div;
+      const double a0 = S2 * div;
+      if (coef) {
+        coef[q] = a3;
+        coef[n + q] = a2;
+        coef[2 * n + q] = a1;
+        coef[3 * n + q] = a0;
+      }
+      /* resolvent cubic y^3 + i1 y^2 + i2 y + i3, largest real root ix1 */
+      const double i1 = -a2;
+      const double i2 = a3 * a1 - 4.0 * a0;
+      const double i3 = 4.0 * a2 * a0 - a1 * a1 - a3 * a3 * a0;
+      const double iR = (9.0 * i1 * i2 - 27.0 * i3 - 2.0 * i1 * i1 * i1) / 54.0;
+      const double iS = (3.0 * i2 - a2 * a2) / 9.0;
+      const double iT = iR * iR + iS * iS * iS;
+      double ix1;
+      if (iT < 0.)
+        ix1 = 2.0 * pow(sqrt(iR * iR + iT), oo3) * cos(atan2(sqrt(-iT), iR) * oo3) - i1 * oo3;
+      else
+        ix1 = pow(iR + sqrt(iT), oo3) + pow(iR - sqrt(iT), oo3) - i1 * oo3;
+      /* the quadratic factor v^2 + iB v + iC holding the physical root */
+      const double iB = 0.5 * (a3 + sqrt(a3 * a3 - 4.0 * a2 + 4.0 * ix1));
+      const double iC = 0.5 * (ix1 - sqrt(ix1 * ix1 - 4.0 * a0));
+      v = (-iB + sqrt(iB * iB - 4.0 * iC)) * 0.5;
+    }
+    v = fmax(v, 0.0);
+    v = fmin(v, 1.0 - 1.0e-15);
+    double div = v / S;
+    if (fabs(S) < 1e-20) div = 0.0;
+    const double rho = sqrt(1.0 - v * v) * D;
+    const double vx = conSx * div, vy = conSy * div, vz = conSz * div;
+    const double p = gammamo * ((E - conSx * vx - conSy * vy - conSz * vz) - rho);
+    const double epsl = p / (gammamo * rho);
+    if (epsl < 0.) {
+      status[q] = OB_C2P_FAIL;
+      continue;
+    }
+    div = 1. / (conD + p + conT);
+    prim[q] = rho;
+    prim[n + q] = epsl;
+    prim[5 * n + q] = p;
+    prim[2 * n + q] = conSix * div;
+    prim[3 * n + q] = conSiy * div;
+    prim[4 * n + q] = conSiz * div;
+  }
 }

@@ -167,6 +167,10 @@ int ob_c2p_bracketed_pt(const ob_c2p_params *P, double D, double S2, double T, d
                         double *rho, double *eps, double *p, int *iters);
 void ob_c2p_bracketed(const ob_c2p_params *P, long n, const double *cons, const double *metric,
                       double *prim, int *status, int *iters);
+/* analytic Gamma-law recovery grhd_c2p_vanal (grhd_c2p.c:239-379); coef
+ * (optional, 4*n): the quartic's a3, a2, a1, a0 per point */
+void ob_c2p_vanal(const ob_c2p_params *P, long n, const double *cons, const double *metric,
+                  double *prim, int *status, double *coef);
 
 /* GRHD flux divergence (grhd_flux_oracle.c): matter_fluxes with
  * grhd_flx1d_fvpr.  eos 0 ideal gas, 1 polytrope; riemann 0 LLF, 1 HLL;

@@ -220,7 +220,11 @@ def test_coupled_rhs_with_atmosphere_matches_oracle():
     _close(got_up[25:], want_up[25:], 1e-14, "u_p conserved after the atmosphere", ref=up30[25:])
     _close(got_u[25:], want_u[25:], 1e-14, "u_c conserved after the c2p rewrite", ref=u30[25:])
     _close(got_prim, want_prim, 1e-12, "primitives")
-    _close(got_r[:25], want_r[:25], 1e-10, "Z4c rhs")
+    # the Z4c right-hand sides of the near-stationary star are small (A_ij:
+    # ~1e-5) differences of O(1) curvature terms, whose rounding floor is
+    # about eps max|u| / h^2 ~ 1e-15 here: the scale of each field is at
+    # least 1e-4, i.e. an absolute bar of 1e-14
+    _close(got_r[:25], want_r[:25], 1e-10, "Z4c rhs", ref=np.maximum(np.abs(want_r[:25]), 1e-4))
     _close(got_r[25:], want_r[25:], 1e-10, "matter rhs")
 
 

@@ -130,3 +130,71 @@ def test_learned_guess_polish_matches_bracketed_and_saves_iterations():
     for a in (dc, dpr, ds, di, fb, train["feat"], train["tgt"], train["g"], test["feat"], test["tgt"], test["g"]):
         a.free()
     net.free()
+
+
+def test_learned_hybrid_matches_reference_hybrid():
+    """MATTER.c2p with the network (bamhip_grhd_c2p_learned on a level)
+    against the oracle's reference hybrid on the same inputs and previous
+    pressures: the C4 states, plus atmosphere-level and cold states.  Where
+    both converge the root is the same within the Newton tolerance and the
+    inversion's conditioning; statuses and conserved rewrites agree."""
+    train = _solved(1 << 18, seed=41)
+    nrm = _norm(train["feat"], train["tgt"])
+    net = B.Mlp(seed=4)
+    net.set(None, nrm)
+    net.train(train["feat"], train["tgt"], epochs=30, batch=2048, lr=3e-3, seed=5)
+    import oracle as O
+    n = 1 << 15
+    prim5, g, W = sample_prims(n, seed=47)
+    Po = O.c2p_params(errormax=1e-12, countmax=100, rhoatm=1e-12)
+    cons = O.p2c(Po, prim5, g)
+    cons[:, :256] *= 1e-6            # atmosphere-level densities
+    cons[1, 256:512] *= -2.0         # negative tau: the cold branch
+    pprev = (Po.gamma - 1) * prim5[0] * prim5[1] * 1.3      # previous pressure, 30% off
+    cp, cs, ci, cc = O.c2p(Po, cons, g, pprev, rhoguess=prim5[0], method="hybrid")
+    lev = B.Level([B.make_box(n, 1, 1, 0.0, 0.0, 0.0, 1.0)], 20)
+    lev.enable(range(20))
+    for k in range(5):
+        lev.put(k, cons[k])
+    for k in range(6):
+        lev.put(5 + k, g[k])
+    prim0 = np.zeros((8, n))
+    prim0[0], prim0[5] = prim5[0], pprev
+    for k in range(8):
+        lev.put(11 + k, prim0[k])
+    ds, di = B.DeviceArray((n,), np.int32), B.DeviceArray((n,), np.int32)
+    fb = B.DeviceArray.from_host(np.zeros(1, np.int32))
+    Pg = B.c2p_params(errormax=1e-12, countmax=100, rhoatm=1e-12)
+    lev.grhd_c2p_learned(Pg, net, list(range(5)), list(range(5, 11)), list(range(11, 19)), ds, di, fb, polish_max=8)
+    st, it = ds.get(), di.get()
+    gp = np.array([lev.get(11 + k) for k in range(8)])
+    gc = np.array([lev.get(k) for k in range(5)])
+    nfb = int(fb.get()[0])
+    lev.free()
+    for a in (ds, di, fb, train["feat"], train["tgt"], train["g"]):
+        a.free()
+    net.free()
+    # status differences: only points the reference's start from the 30%-off
+    # pressure loses (Newton failure, or an iterate dipping below the
+    # atmosphere threshold) and the learned start solves
+    diff = st != cs
+    assert np.all(((cs[diff] == O.C2P_NR) | (cs[diff] == O.C2P_ATM)) & (st[diff] == O.C2P_OK)), \
+        (np.unique(cs[diff]), np.unique(st[diff]))
+    assert diff.mean() < 0.05
+    both = (st == O.C2P_OK) & (cs == O.C2P_OK)
+    assert both.mean() > 0.9
+    # Newton on both sides stops once the relative update is below 1e-12
+    # (quadratic convergence: the iterate is then at the root to rounding);
+    # the root is resolved to the inversion's conditioning W^2 (tau+D)/p
+    kappa = W ** 2 * (cons[1] + cons[0]) / np.maximum(cp[5], 1e-300)
+    bound = np.maximum(64 * np.finfo(float).eps * kappa, 1e-12)
+    assert np.all(np.abs(gp[5] / cp[5] - 1)[both] <= bound[both])
+    assert np.all(np.abs(gp[0] / cp[0] - 1)[both] <= bound[both])
+    # atmosphere and cold points follow the reference exactly (no network)
+    same = (cs == O.C2P_ATM) | (cs == O.C2P_COLD)
+    assert same.sum() >= 256
+    np.testing.assert_array_equal(gp[:, same & (st == cs)], cp[:, same & (st == cs)])
+    np.testing.assert_array_equal(gc[:, same & (st == cs)], cc[:, same & (st == cs)])
+    print("learned hybrid: %d fallbacks of %d points, mean iterations %.2f (reference hybrid %.2f)"
+          % (nfb, n, it.mean(), ci.mean()), flush=True)
+    assert nfb <= 1e-2 * n

@@ -216,3 +216,117 @@ def test_atmosphere_reset_of_upre():
     cons2[2, 10] = np.nan
     assert O.grhd_atm_set_u(P, cons2, prim2) == 1
     assert np.all(cons2[:, 10] == 0.0) and prim2[7, 10] == 1.0
+
+
+# ------------------------------------------------ analytic recovery (vanal)
+def _vanal_kappa(coef, v):
+    """conditioning of rho = sqrt(1 - v^2) D through the quartic's root:
+    (sum_k |a_k v^k|) / (v |q'(v)|) * v^2 / (1 - v^2), plus 1"""
+    a3, a2, a1, a0 = coef
+    q1 = 4 * v ** 3 + 3 * a3 * v ** 2 + 2 * a2 * v + a1
+    sc = v ** 4 + np.abs(a3) * v ** 3 + np.abs(a2) * v ** 2 + np.abs(a1) * v + np.abs(a0)
+    return sc / np.abs(q1) * v / (1 - v * v) + 1
+
+
+def test_vanal_quartic_vanishes_at_the_true_speed():
+    """flat metric: the quartic built from (D, tau, S) has the original |v|
+    as a root, to rounding of its terms (a dropped term or a wrong sign in
+    a3..a0 leaves an O(1) residual)"""
+    prim5, g, _ = sample_prims(20000, seed=3, metric_noise=0.0)
+    P = O.c2p_params(**P12)
+    cons = O.p2c(P, prim5, g)
+    _, st, coef = O.c2p_vanal(P, cons, g)
+    assert np.all(st == O.C2P_OK)
+    v = np.sqrt((prim5[2:5] ** 2).sum(axis=0))
+    a3, a2, a1, a0 = coef
+    res = v ** 4 + a3 * v ** 3 + a2 * v ** 2 + a1 * v + a0
+    sc = v ** 4 + np.abs(a3) * v ** 3 + np.abs(a2) * v ** 2 + np.abs(a1) * v + np.abs(a0)
+    assert np.max(np.abs(res) / sc) < 1e-14
+
+
+def test_vanal_root_against_high_precision():
+    """the Ferrari closed form (resolvent cubic, quadratic factor) picks the
+    physical root: rho against sqrt(1 - v^2) D with v from a 50-digit
+    polynomial root of the same FP64 coefficients, within 32 eps times the
+    root's conditioning"""
+    mp.mp.dps = 50
+    prim5, g, _ = sample_prims(300, seed=5, metric_noise=0.0)
+    P = O.c2p_params(**P12)
+    cons = O.p2c(P, prim5, g)
+    prim, st, coef = O.c2p_vanal(P, cons, g)
+    assert np.all(st == O.C2P_OK)
+    vtrue = np.sqrt((prim5[2:5] ** 2).sum(axis=0))
+    for q in range(prim5.shape[1]):
+        a = [mp.mpf(float(c)) for c in coef[:, q]]
+        roots = mp.polyroots([1] + a, maxsteps=200, extraprec=200)
+        real = [r.real for r in roots if abs(r.imag) < mp.mpf(10) ** -30 and 0 <= r.real < 1]
+        v = min(real, key=lambda r: abs(r - vtrue[q]))
+        rho = float(mp.sqrt(1 - v * v) * mp.mpf(float(cons[0, q])))
+        kap = _vanal_kappa(coef[:, q], float(v))
+        assert abs(prim[0, q] - rho) <= 32 * np.finfo(float).eps * kap * rho, q
+
+
+def test_vanal_static_fluid_closed_form():
+    """S = 0: v = 0, rho = D, p = (Gamma - 1) tau, eps = tau / D, v^i = 0;
+    v^2 (prim[6]) is left as it was"""
+    n = 64
+    rng = np.random.default_rng(4)
+    cons = np.zeros((5, n))
+    cons[0] = 10.0 ** rng.uniform(-8, -2, n)
+    cons[1] = cons[0] * 10.0 ** rng.uniform(-4, 0, n)
+    g = np.zeros((6, n))
+    g[0] = g[3] = g[5] = 1.0
+    prim0 = np.full((8, n), 7.0)
+    P = O.c2p_params(**P12)
+    prim, st, coef = O.c2p_vanal(P, cons, g, prim0=prim0)
+    assert np.all(st == O.C2P_OK) and np.all(coef == 0.0)
+    np.testing.assert_array_equal(prim[0], cons[0])
+    # p = (Gamma - 1)((tau + D) - D): the rounding of tau + D is relative to D
+    tol = 4 * np.finfo(float).eps * (cons[0] + cons[1]) / cons[1]
+    assert np.all(np.abs(prim[5] - (P.gamma - 1) * cons[1]) <= tol * (P.gamma - 1) * cons[1])
+    assert np.all(np.abs(prim[1] - cons[1] / cons[0]) <= 2 * tol * cons[1] / cons[0])
+    assert np.all(prim[2:5] == 0.0) and np.all(prim[6] == 7.0) and np.all(prim[7] == 1.0)
+
+
+def test_vanal_conformally_flat_pressure_term():
+    """on gamma_ij = a delta_ij the reference forms S_i v_i with the lowered
+    S_i and v_i = S_i |v| / |S|, i.e. a |S| |v| instead of |S| |v|: its
+    pressure is the true one minus (Gamma - 1)(a - 1)|S||v| (rho and |v|,
+    from invariants, are exact)"""
+    a = 1.01
+    prim5, g, _ = sample_prims(2000, seed=8, metric_noise=0.0)
+    g[0] = g[3] = g[5] = a
+    # contravariant v^i with a |v^i|^2 = 1 - 1/W^2
+    prim5[2:5] /= np.sqrt(a)
+    P = O.c2p_params(**P12)
+    cons = O.p2c(P, prim5, g)
+    prim, st, _ = O.c2p_vanal(P, cons, g)
+    ok = st == O.C2P_OK
+    assert ok.sum() > 100
+    S = np.sqrt((cons[2:5] ** 2).sum(axis=0) / a) / a ** 1.5     # |S| = sqrt(S_i S^i), undensitised
+    v = np.sqrt(a * (prim5[2:5] ** 2).sum(axis=0))
+    ptrue = (P.gamma - 1) * prim5[0] * prim5[1]
+    expect = ptrue - (P.gamma - 1) * (a - 1) * S * v
+    big = ok & (np.abs(expect) > 1e-3 * (cons[1] + cons[0]) / a ** 1.5)
+    np.testing.assert_allclose(prim[5][big], expect[big], rtol=1e-6)
+    np.testing.assert_allclose(prim[0][ok], prim5[0][ok], rtol=1e-8)
+    # where the term drives the pressure negative the reference stops (eps < 0)
+    scale = 1e-10 * (cons[0] + cons[1]) / a ** 1.5
+    assert np.all(st[expect < -scale] == O.C2P_FAIL) and np.all(st[expect > scale] == O.C2P_OK)
+
+
+def test_vanal_agrees_with_the_newton_solvers_flat():
+    """an independent method on the same inputs: rho from the analytic root
+    and from the hybrid p-root (tolerance 1e-12) agree within the two
+    methods' conditioning"""
+    prim5, g, _ = sample_prims(20000, seed=9, metric_noise=0.0)
+    P = O.c2p_params(**P12)
+    cons = O.p2c(P, prim5, g)
+    pa, sa, coef = O.c2p_vanal(P, cons, g)
+    ph, sh, _, _ = O.c2p(P, cons, g, np.zeros(cons.shape[1]), rhoguess=prim5[0], method="bracketed")
+    ok = (sa == O.C2P_OK) & (sh == O.C2P_OK)
+    assert ok.mean() > 0.99
+    v = np.sqrt((prim5[2:5] ** 2).sum(axis=0))
+    kap = _vanal_kappa(coef, v)
+    rel = np.abs(pa[0] - ph[0]) / ph[0]
+    assert np.all(rel[ok] <= 1e-12 + 64 * np.finfo(float).eps * kap[ok])
