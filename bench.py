@@ -18,10 +18,17 @@ Workloads (synthetic data of the configurations' shapes, FP64 throughout):
   grid-point updates per second, sum over levels of 2^l x interior points.
 * box192: one 192^3 periodic box (+3 ghosts), flat space + 1e-10 noise,
   full RK4 steps; it times the Z4c RHS bulk kernel for the `roofline`.
+* c1: the single-box configuration (BL m = 1, 64^3 interior on [-8, 8]^3,
+  8 RK4 steps), on the device and with the CPU oracle.
+* box192 also runs the robust-stability test: 100 RK4 steps from the noise
+  data with the Hamiltonian constraint's L2 norm every 10 steps.
 * c2p: con2prim on 2^24 points of the con2prim distribution (ideal gas,
   Gamma 2, tolerance 1e-12): the reference's plain p-root, the bracketed
   solver from a cold start, and the learned guess (3-64-64-1 network trained
-  on the GPU on 2^22 solved samples) polished by the bracketed solver.
+  on the GPU on 2^22 solved samples) polished by the bracketed solver; the
+  analytic recovery grhd_c2p_vanal against the original primitives (on the
+  distribution's metric and on a flat one); the CPU oracle's hybrid and
+  bracketed solvers on 2^20 points.
 
 `roofline` is for the dominant kernel, z4c_rhs_bulk: achieved = 5.5e3
 algorithmic FP64 FLOP per point x points per launch / the launch's average
@@ -63,7 +70,7 @@ def parse():
     ap.add_argument("--rk-fusion", type=int, default=2,
                     help="RK4 stage update inside the RHS kernels: 0 never, 1 always, 2 on levels >= 2^22 points")
     ap.add_argument("--xcd-map", type=int, default=1, help="1: XCD-aware tile numbering in the RHS kernels")
-    ap.add_argument("--only", default="fmr,box192,c2p,c5")
+    ap.add_argument("--only", default="fmr,box192,c1,c2p,c5")
     return ap.parse_args()
 
 
@@ -262,15 +269,24 @@ def run_box(args):
     B.synchronize()
     B.timer_enable(False)
     ms, cnt = B.timer_get("z4_rhs")
+    # the robust-stability run: 100 RK4 steps from the noise data, the
+    # Hamiltonian constraint's L2 norm every 10 steps
+    lev.put_state(U, flat_noise(nst, g, seed=20261202).reshape(25, -1))
+    ham_hist = [lev.z4c_constraints_l2(U)[0]]
+    for step in range(1, 101):
+        lev.z4c_rk4_step(par, dt, U, P, R, V, pgh=g)
+        if step % 10 == 0:
+            ham_hist.append(lev.z4c_constraints_l2(U)[0])
     lev.free()
     npts = n ** 3
-    return dict(value=npts * args.box_steps / (t1 - t0), ms_per_step=1e3 * (t1 - t0) / args.box_steps,
+    return dict(ham_l2_every_10_steps=[float("%.6g" % x) for x in ham_hist],
+                value=npts * args.box_steps / (t1 - t0), ms_per_step=1e3 * (t1 - t0) / args.box_steps,
                 rhs_launch_ms=ms / max(cnt, 1), npts=npts, ham_mom_l2_after=[float(x) for x in l2],
                 breakdown_ms_per_step={k: v[0] / args.box_steps for k, v in t.items()})
 
 
 # ------------------------------------------------------------------ c2p
-def c2p_prims(n, seed):
+def c2p_prims(n, seed, metric_noise=1e-3):
     """rho log-uniform [1e-10, 1e-2], eps log-uniform [1e-6, 1], W uniform
     [1, 10] in a random direction, metric flat + 1e-3 seeded perturbation"""
     rng = np.random.default_rng(seed)
@@ -281,7 +297,7 @@ def c2p_prims(n, seed):
     d /= np.linalg.norm(d, axis=0)
     gm = np.zeros((6, n))
     gm[0] = gm[3] = gm[5] = 1.0
-    gm += 1e-3 * rng.uniform(-1, 1, (6, n))
+    gm += metric_noise * rng.uniform(-1, 1, (6, n))
     gdd = (gm[0] * d[0] ** 2 + gm[3] * d[1] ** 2 + gm[5] * d[2] ** 2
            + 2 * (gm[1] * d[0] * d[1] + gm[2] * d[0] * d[2] + gm[4] * d[1] * d[2]))
     v = np.sqrt((1.0 - 1.0 / W ** 2) / gdd) * d
@@ -372,6 +388,9 @@ def run_c2p(args):
                           fallback_fraction=float(fb.get()[0]) / n, max_rel_diff_p_vs_bracketed=agree,
                           training=dict(samples=ntr, epochs=30, batch=4096, seconds=train_s,
                                         loss_first=float(losses[0]), loss_last=float(losses[-1])))
+    # the analytic recovery grhd_c2p_vanal as a cross-check of the primitives
+    out["vanal"] = {name: vanal_report(B, P, 1 << 22, 4343, noise)
+                    for name, noise in (("c4_metric", 1e-3), ("flat_metric", 0.0))}
     out["learned_vs_plain_speedup"] = out["learned"]["pts_per_s"] / out["plain"]["pts_per_s"]
     out["learned_vs_bracketed_speedup"] = out["learned"]["pts_per_s"] / out["bracketed"]["pts_per_s"]
     for a in (dc0, dg, dc, dpr, ds, di, fb):
@@ -542,6 +561,138 @@ def run_c5(args, net):
     return out
 
 
+def vanal_report(B, P, n, seed, metric_noise):
+    """grhd_c2p_vanal on the device for n points of the con2prim
+    distribution: the fraction whose (rho, eps, v^i) match the original
+    primitives to 1e-10, the fraction where the recovery stops (eps < 0),
+    and of the rest the fraction within 64 eps times the conditioning of
+    the quartic's root (kappa_v v^2 / (1 - v^2) for rho, times the
+    cancellation (E + |S||v| + rho) / |E - |S||v| - rho| for eps)"""
+    prim5, gm = c2p_prims(n, seed, metric_noise)
+    dp, dg = B.DeviceArray.from_host(prim5), B.DeviceArray.from_host(gm)
+    dc = B.DeviceArray((5, n))
+    B.p2c_batch(P, dp, dg, dc)
+    dpr = B.DeviceArray.from_host(np.zeros((8, n)))
+    ds, di = B.DeviceArray((n,), np.int32), B.DeviceArray((n,), np.int32)
+    B.synchronize()
+    t0 = time.perf_counter()
+    B.c2p_batch(P, B.C2P_VANAL, dc, dg, dpr, ds, di)
+    B.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0)
+    cons, pr, st = dc.get(), dpr.get(), ds.get()
+    for a in (dp, dg, dc, dpr, ds, di):
+        a.free()
+    ok = st == B.C2P_OK
+    rel = lambda a, b: np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+    vmag = np.sqrt((prim5[2:5] ** 2).sum(axis=0))
+    err_rho, err_eps = rel(pr[0], prim5[0]), rel(pr[1], prim5[1])
+    err_v = np.abs(pr[2:5] - prim5[2:5]).max(axis=0) / np.maximum(vmag, 1e-300)
+    match = ok & (err_rho <= 1e-10) & (err_eps <= 1e-10) & (err_v <= 1e-10)
+    # conditioning from the invariants (undensitised D, E = tau + D, |S|)
+    g = gm
+    det = g[0] * (g[3] * g[5] - g[4] ** 2) - g[1] * (g[1] * g[5] - g[2] * g[4]) + g[2] * (g[1] * g[4] - g[2] * g[3])
+    sq = np.sqrt(det)
+    D, E = cons[0] / sq, (cons[0] + cons[1]) / sq
+    W = 1 / np.sqrt(1 - (g[0] * prim5[2] ** 2 + g[3] * prim5[3] ** 2 + g[5] * prim5[4] ** 2 + 2 * (
+        g[1] * prim5[2] * prim5[3] + g[2] * prim5[2] * prim5[4] + g[4] * prim5[3] * prim5[4])))
+    v = np.sqrt(1 - 1 / W ** 2)
+    h = 1 + prim5[1] + (P.gamma - 1) * prim5[1]
+    S = prim5[0] * h * W ** 2 * v
+    G, gmo = P.gamma, P.gamma - 1
+    div = 1 / (gmo ** 2 * (S ** 2 + D ** 2))
+    a3, a2 = -2 * G * gmo * S * E * div, (G * G * E * E + 2 * gmo * S ** 2 - gmo ** 2 * D ** 2) * div
+    a1, a0 = -2 * G * S * E * div, S ** 2 * div
+    q1 = 4 * v ** 3 + 3 * a3 * v ** 2 + 2 * a2 * v + a1
+    kv = (v ** 4 + np.abs(a3) * v ** 3 + np.abs(a2) * v ** 2 + np.abs(a1) * v + np.abs(a0)) / np.maximum(
+        np.abs(q1) * v, 1e-300) + 1
+    eps_m = np.finfo(float).eps
+    krho = 1 + kv * v * v / np.maximum(1 - v * v, 1e-300)
+    keps = krho * (E + S * v + D / W) / np.maximum(np.abs(E - S * v - D / W), 1e-300)
+    rest = ok & ~match
+    within = rest & (err_rho <= 64 * eps_m * krho) & (err_eps <= 64 * eps_m * keps)
+    return dict(n=n, metric_noise=metric_noise, ms=ms, match_1e_10=float(match.mean()),
+                stopped_eps_negative=float((st == B.C2P_FAIL).mean()), other=float(rest.mean()),
+                other_within_64_eps_kappa=float(within.sum() / max(rest.sum(), 1)))
+
+
+def run_c1(args):
+    """the single-box configuration: BL m = 1 at the origin, 64^3 interior
+    on [-8, 8]^3 (h = 0.25, cell centred) with one radiative boundary layer,
+    precollapsed lapse, 1+log / Gamma driver, KO 0.1, dt = h/4, 8 RK4 steps
+    on the device, and the same steps with the OpenMP CPU oracle (bounded
+    by the CPU sample time)"""
+    import baum_amd as B
+    h, n = 0.25, 66
+    x = -8.0 - 0.5 * h + h * np.arange(n)
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    psi = 1 + 1.0 / (2 * np.sqrt(X ** 2 + Y ** 2 + Z ** 2))
+    u = np.zeros((25, n, n, n))
+    u[0] = u[3] = u[5] = 1.0
+    u[6] = psi ** -4
+    u[18] = psi ** -2
+    lev = B.Level([B.make_box(n, n, n, x[0], x[0], x[0], h)], 100)
+    lev.enable(range(100))
+    for v in range(100):
+        lev.fill(v, 0.0)
+    U, P, R, V = (list(range(25 * k, 25 * k + 25)) for k in range(4))
+    par = B.z4c_params()
+    dt = 0.25 * h
+    lev.put_state(U, u.reshape(25, -1))
+    lev.z4c_rk4_step(par, dt, U, P, R, V)          # tunes the tile kernels for this box
+    lev.put_state(U, u.reshape(25, -1))
+    B.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(8):
+        lev.z4c_rk4_step(par, dt, U, P, R, V)
+    B.synchronize()
+    el = time.perf_counter() - t0
+    l2 = lev.z4c_constraints_l2(U)
+    lev.free()
+    out = dict(value=8 * 64 ** 3 / el, unit="grid-point-updates/s", ms_per_step=1e3 * el / 8,
+               ham_mom_l2_after_8_steps=[float("%.6g" % x) for x in l2])
+    if not args.no_cpu:
+        import oracle as O
+        ob = O.make_box(n, n, n, x[0], x[0], x[0], h)
+        work = np.zeros((75, n, n, n))
+        steps, t0 = 0, time.perf_counter()
+        while steps < 8:
+            O.z4c_rk4_step(ob, O.z4c_params(), dt, u, bc=1, pgh=0, work=work)
+            steps += 1
+            if time.perf_counter() - t0 >= args.cpu_seconds:
+                break
+        el = time.perf_counter() - t0
+        out["cpu"] = dict(value=steps * 64 ** 3 / el, unit="grid-point-updates/s",
+                          cores=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), kind="port",
+                          sample="%d RK4 steps of the 64^3 box with the OpenMP C oracle, %.1f s" % (steps, el))
+    return out
+
+
+def c4_cpu_baseline(target_s):
+    """the OpenMP CPU oracle's con2prim (the reference hybrid and the
+    bracketed solver) on a bounded sample of the con2prim distribution"""
+    import baum_amd as B
+    import oracle as O
+    n = 1 << 20
+    prim5, gm = c2p_prims(n, 4242)
+    Po = O.c2p_params(errormax=1e-12, countmax=100, rhoatm=1e-20)
+    cons = O.p2c(Po, prim5, gm)
+    out = {}
+    for name in ("hybrid", "bracketed"):
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            _, st, it, _ = O.c2p(Po, cons, gm, np.zeros(n), rhoguess=prim5[0], method=name)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= target_s / 2 or reps >= 20:
+                break
+        out[name] = dict(pts_per_s=reps * n / el, converged=float(np.mean(st == B.C2P_OK)),
+                         mean_iters=float(it.mean()),
+                         iter_hist=np.bincount(np.minimum(it, 12), minlength=13).tolist())
+    out.update(cores=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), kind="port",
+               sample="2^20 points of the con2prim distribution (tolerance 1e-12), cold start, OpenMP C oracle")
+    return out
+
+
 # ------------------------------------------------------------------ CPU baseline
 def cpu_baseline(target_s):
     """the OpenMP CPU oracle (test infrastructure, used here only as the
@@ -604,6 +755,8 @@ def main():
     fmr = run_fmr(args) if "fmr" in only else None
     progress("192^3 box")
     box = run_box(args) if "box192" in only else None
+    progress("64^3 puncture box")
+    c1 = run_c1(args) if "c1" in only else None
     if fmr:
         out.update(value=fmr["value"], unit="grid-point-updates/s", ms_per_step=fmr["ms_per_step"])
         out["config"] = {"workload": "Z4c BO evolution of a 7-level FMR hierarchy: 128^3 level 0 + 2 boxes of "
@@ -631,7 +784,10 @@ def main():
                            "workload": "192^3 periodic box (+3 ghosts), one RHS launch"}
         out["box192"] = {"value": box["value"], "unit": "grid-point-updates/s", "ms_per_step": box["ms_per_step"],
                          "breakdown_ms_per_step": box["breakdown_ms_per_step"],
-                         "ham_mom_l2_after": box["ham_mom_l2_after"]}
+                         "ham_mom_l2_after": box["ham_mom_l2_after"],
+                         "ham_l2_every_10_steps_of_100": box["ham_l2_every_10_steps"]}
+    if c1:
+        out["c1"] = c1
     # tile kernel chosen by the autotuner for each tuned box set, with the time of every variant
     out["rhs_tuning"] = B.tune_report().splitlines()
     if "c2p" in only or "c5" in only:
@@ -639,6 +795,9 @@ def main():
         c2p_out, net = run_c2p(args)
         if "c2p" in only:
             out["c2p"] = c2p_out
+            if not args.no_cpu:
+                progress("con2prim CPU baseline")
+                out["c2p"]["cpu"] = c4_cpu_baseline(args.cpu_seconds)
         if "c5" in only:
             progress("coupled Z4c + GRHD on the TOV star")
             out["c5"] = run_c5(args, net)
