@@ -87,5 +87,8 @@ struct TimerScope {
 
 // collect the 25 device pointers of an index list (returns error if missing)
 int gather_ptrs(bamhip_level *lev, const int *idx, int n, double **out, const char *what);
+// looks up the device address of the fixed reduction scratch of output.hip
+// on `device` (current); called once per device by bamhip_init
+int noalloc_scratch_init(int device);
 
 }  // namespace bamhip

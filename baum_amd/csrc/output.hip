@@ -137,6 +137,96 @@ extern "C" int bamhip_level_reduce(bamhip_level *lev, int nv, const int *vars, i
   return BAMHIP_OK;
 }
 
+// ---- the reduction without allocation (bamhip_level_reduce_noalloc)
+// Block partials live in `scratch`, a fixed device array (one copy per GPU,
+// held until the process ends); its device address is looked up once per
+// device when the library is initialised (noalloc_scratch_init, called by
+// bamhip_init with the device current), and the host never passes the
+// array's name itself to a kernel or a copy.  The host-side copy of the
+// partials is a fixed array too.  Every slot a call reads is written by the
+// same call's kernels (one slot per launched block), so nothing needs
+// clearing; calls that overlap on one GPU share the array and race.
+constexpr int kNoallocBlocks = 4096;   // at most 4096 reduction blocks per level
+constexpr int kNoallocVars = 64;
+constexpr int kNoallocBoxes = 64;
+constexpr int kNoallocDevices = 16;
+static __device__ __align__(256) double scratch[kRVals * kNoallocBlocks];
+static double *g_scratch_dev[kNoallocDevices];
+static double g_scratch_host[kRVals * kNoallocBlocks];
+
+namespace bamhip {
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kNoallocDevices)
+    return set_error(BAMHIP_EINVAL, "bamhip_init: device %d beyond the %d supported by the reduction scratch", device,
+                     kNoallocDevices);
+  if (g_scratch_dev[device]) return BAMHIP_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return set_error(BAMHIP_EDEVICE, "bamhip_init: address of the reduction scratch: %s", hipGetErrorString(e));
+  g_scratch_dev[device] = (double *)p;
+  return BAMHIP_OK;
+}
+}  // namespace bamhip
+
+extern "C" int bamhip_level_reduce_noalloc(bamhip_level *lev, int nv, const int *vars, int type, double *result) {
+  const char *what = "bamhip_level_reduce_noalloc";
+  if (!lev || nv < 1 || !vars || !result) return set_error(BAMHIP_EINVAL, "%s: bad arguments", what);
+  if (type < 0 || type > 4) return set_error(BAMHIP_EINVAL, "%s: type %d (0 max, 1 min, 2 norm, 3 norminf, 4 integral)", what, type);
+  if (nv > kNoallocVars) return set_error(BAMHIP_EINVAL, "%s: %d variables, at most %d", what, nv, kNoallocVars);
+  const int nbox = (int)lev->boxes.size();
+  if (nbox > kNoallocBoxes) return set_error(BAMHIP_EINVAL, "%s: %d boxes, at most %d", what, nbox, kNoallocBoxes);
+  int dev = -1;
+  BH_HIP(hipGetDevice(&dev));
+  double *part = (dev >= 0 && dev < kNoallocDevices) ? g_scratch_dev[dev] : nullptr;
+  if (!part) return set_error(BAMHIP_EINVAL, "%s: no reduction scratch on device %d (call bamhip_init)", what, dev);
+  double *u[kNoallocVars];
+  int e;
+  if ((e = gather_ptrs(lev, vars, nv, u, what))) return e;
+  // the block layout of bamhip_level_reduce, so both give the same sums
+  long ninner[kNoallocBoxes], nblk[kNoallocBoxes];
+  long totblk = 0, npts = 0;
+  for (int b = 0; b < nbox; b++) {
+    const DevBox &B = lev->boxes[b];
+    const long ni = B.m - B.skip[0] - B.skip[1], nj = B.n - B.skip[2] - B.skip[3], nk = B.o - B.skip[4] - B.skip[5];
+    ninner[b] = (ni > 0 && nj > 0 && nk > 0) ? ni * nj * nk : 0;
+    nblk[b] = ninner[b] ? std::min<long>(kRMaxBlocks, (ninner[b] + kRBlock - 1) / kRBlock) : 0;
+    totblk += nblk[b];
+    npts += ninner[b];
+  }
+  if (totblk == 0) return set_error(BAMHIP_EINVAL, "%s: the level has no inner points", what);
+  if (totblk > kNoallocBlocks)
+    return set_error(BAMHIP_EINVAL, "%s: the level needs %ld reduction blocks, the fixed scratch holds %d", what, totblk,
+                     kNoallocBlocks);
+  for (int v = 0; v < nv; v++) {
+    long off = 0;
+    for (int b = 0; b < nbox; b++) {
+      if (!nblk[b]) continue;
+      reduce_kernel<<<(unsigned)nblk[b], kRBlock, 0, stream()>>>(u[v], lev->boxes[b], ninner[b], part + kRVals * off);
+      off += nblk[b];
+    }
+    BH_HIP(hipGetLastError());
+    BH_HIP(hipMemcpyAsync(g_scratch_host, part, sizeof(double) * kRVals * totblk, hipMemcpyDeviceToHost, stream()));
+    BH_HIP(hipStreamSynchronize(stream()));
+    double mx = -DBL_MAX, mn = DBL_MAX, sum = 0., sq = 0., ma = 0.;
+    for (long k = 0; k < totblk; k++) {
+      const double *p = &g_scratch_host[kRVals * k];
+      mx = p[0] > mx ? p[0] : mx;
+      mn = p[1] < mn ? p[1] : mn;
+      sum += p[2];
+      sq += p[3];
+      ma = p[4] > ma ? p[4] : ma;
+    }
+    const double n = (double)npts * nv;
+    if (type == 0) result[v] = mx;
+    if (type == 1) result[v] = mn;
+    if (type == 2) result[v] = sqrt(sq / n) * sqrt((double)nv);
+    if (type == 3) result[v] = ma;
+    if (type == 4) result[v] = sum * pow(lev->boxes[0].dx, 3);
+  }
+  return BAMHIP_OK;
+}
+
 extern "C" int bamhip_level_gather(bamhip_level *lev, int var, long n, const long *idx, double *out) {
   const char *what = "bamhip_level_gather";
   if (!lev || n < 0 || (n > 0 && (!idx || !out))) return set_error(BAMHIP_EINVAL, "%s: bad arguments", what);

@@ -136,6 +136,8 @@ int bamhip_init(int device) {
   BH_HIP(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return set_error(BAMHIP_EINVAL, "bamhip_init: device %d of %d", device, n);
   BH_HIP(hipSetDevice(device));
+  int e;
+  if ((e = noalloc_scratch_init(device))) return e;
   if (!g_stream || g_device != device) {
     if (g_stream) (void)hipStreamDestroy(g_stream);
     if (g_side) (void)hipStreamDestroy(g_side);

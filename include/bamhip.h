@@ -94,6 +94,17 @@ int bamhip_level_box(const bamhip_level *lev, int b, bamhip_box *box, long *noff
  * the count of all nv variables, times sqrt(nv)), 3 max |u|, 4 sum * dx^3.
  * result[nv] on the host; deterministic (fixed blocks, fixed sum order). */
 int bamhip_level_reduce(bamhip_level *lev, int nv, const int *vars, int type, double *result);
+/* bamhip_level_reduce without any allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels, block layout and results (bitwise) as
+ * bamhip_level_reduce; the block partials live in a fixed device array set
+ * up by bamhip_init on each device and kept until the process ends, so a
+ * level may need at most 4096 reduction blocks (each box takes
+ * min(1024, ceil(inner points / 256))), at most 64 boxes and 64 variables;
+ * larger calls return BAMHIP_EINVAL.  Calls that overlap on one device share
+ * the array and race. */
+int bamhip_level_reduce_noalloc(bamhip_level *lev, int nv, const int *vars, int type, double *result);
 /* host out[t] = var[idx[t]] for n level indices (bufferorderedpoints) */
 int bamhip_level_gather(bamhip_level *lev, int var, long n, const long *idx, double *out);
 
