@@ -464,6 +464,37 @@ __global__ void c2p_bracketed_kernel(C2PArrays A, C2PK P, long n) {
   }
 }
 
+// MATTER.c2p with the bracketed solver: the hybrid of grhd_c2p_v2.c:265-464
+// whose hot solve is the bracketed Newton iteration from the previous
+// pressure.  A state with D < fatm rhoatm (atmosphere whatever the
+// pressure) and a state whose bracketed solve is not OK, or lands below the
+// atmosphere threshold, at eps < 0 or beyond vmax, take the reference's hot
+// solve instead, so the cold branch, the atmosphere and the conserved
+// rewrites are the reference's.
+__global__ void c2p_bracketed_hybrid_kernel(C2PArrays A, C2PK P, long n) {
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    HybPoint h;
+    hybrid_prepare(A, q, h);
+    double rho = 0, epsl = 0, pres = 0;
+    int it = 0;
+    int result = h.result;
+    if (result != C2P_FAIL) {
+      bool done = false;
+      if (h.conD >= P.fatm * P.rhoatm) {
+        result = c2p_bracketed_pt(P, h.conD, h.conS2, h.conT, A.prim[5][q], &rho, &epsl, &pres, &it, P.countmax);
+        const double s = h.conT + h.conD + pres;
+        done = result == C2P_OK && rho >= P.fatm * P.rhoatm && epsl >= 0. && h.conS2 / (s * s) <= P.vmax;
+      }
+      if (!done) {
+        int it2 = 0;
+        result = try_hot(P, h.conD, h.conS2, h.conT, &rho, &epsl, &pres, A.prim[5][q], &it2);
+        it += it2;
+      }
+    }
+    hybrid_finish(A, P, q, h, result, rho, epsl, pres, it);
+  }
+}
+
 // Learned guess + bracketed polish: each wavefront evaluates the network
 // for its 64 points on the matrix cores (mlp_device.h), then every lane
 // polishes its point with the bracketed Newton solver started at the
@@ -760,7 +791,10 @@ static C2PK make_c2pk(const bamhip_c2p_params *p) {
   return K;
 }
 
-int c2p_launch(const bamhip_c2p_params *par, int method, long n, C2PArrays &A) {
+// level: MATTER.c2p on a level, where the bracketed method takes the
+// hybrid's control flow (c2p_bracketed_hybrid_kernel); flat arrays run the
+// bracketed solver alone
+int c2p_launch(const bamhip_c2p_params *par, int method, long n, C2PArrays &A, bool level) {
   if (!par || n < 0) return set_error(BAMHIP_EINVAL, "bamhip_c2p: bad arguments");
   if (par->eos != 0 && par->eos != 1) return set_error(BAMHIP_EINVAL, "bamhip_c2p: eos=%d (0 ideal, 1 poly)", par->eos);
   if (!(par->gamma > 1.0)) return set_error(BAMHIP_EINVAL, "bamhip_c2p: gamma must be > 1");
@@ -774,6 +808,8 @@ int c2p_launch(const bamhip_c2p_params *par, int method, long n, C2PArrays &A) {
     c2p_hybrid_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(A, K, n);
   else if (method == BAMHIP_C2P_PLAIN)
     c2p_plain_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(A, K, n);
+  else if (method == BAMHIP_C2P_BRACKETED && level)
+    c2p_bracketed_hybrid_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(A, K, n);
   else if (method == BAMHIP_C2P_BRACKETED)
     c2p_bracketed_kernel<<<(unsigned)blocks, 256, 0, stream()>>>(A, K, n);
   else if (method == BAMHIP_C2P_VANAL) {
@@ -799,7 +835,7 @@ extern "C" int bamhip_c2p_batch(const bamhip_c2p_params *par, int method, long n
   for (int k = 0; k < 8; k++) A.prim[k] = prim + k * n;
   A.status = status;
   A.iters = iters;
-  return c2p_launch(par, method, n, A);
+  return c2p_launch(par, method, n, A, false);
 }
 
 static void c2p_flat(C2PArrays &A, long n, double *cons, const double *metric, double *prim, int *status,
@@ -934,5 +970,5 @@ extern "C" int bamhip_grhd_c2p(bamhip_level *lev, const bamhip_c2p_params *par, 
   for (int k = 0; k < 8; k++) A.prim[k] = tmp[k];
   A.status = status;
   A.iters = iters;
-  return c2p_launch(par, method, lev->npoints, A);
+  return c2p_launch(par, method, lev->npoints, A, true);
 }

@@ -125,6 +125,43 @@ TimerScope::~TimerScope() {
   if (g_pending.size() > 4096) resolve_pending();
 }
 
+// Device arrays of grid functions are recycled: a variable freed with its
+// level (or disabled) goes to a pool keyed by its size, and enabling a
+// variable takes an array of that size from the pool before calling
+// hipMalloc.  A regrid (move_box) frees a level and creates one of the same
+// box sizes, so its ~100 variables per level cost no allocation.  The pool
+// keeps at most kPoolMax bytes; arrays beyond that are freed.  Every array
+// entering the pool is idle: level_free and level_disable synchronise the
+// library stream first.
+static std::multimap<size_t, double *> g_arrays;
+static size_t g_array_bytes = 0;
+constexpr size_t kPoolMax = size_t(32) << 30;
+
+static double *array_get(size_t bytes) {
+  auto it = g_arrays.find(bytes);
+  if (it == g_arrays.end()) return nullptr;
+  double *p = it->second;
+  g_arrays.erase(it);
+  g_array_bytes -= bytes;
+  return p;
+}
+
+static void array_put(double *p, size_t bytes) {
+  if (!p) return;
+  if (g_array_bytes + bytes > kPoolMax) {
+    (void)hipFree(p);
+    return;
+  }
+  g_arrays.emplace(bytes, p);
+  g_array_bytes += bytes;
+}
+
+static void array_pool_release() {
+  for (auto &kv : g_arrays) (void)hipFree(kv.second);
+  g_arrays.clear();
+  g_array_bytes = 0;
+}
+
 }  // namespace bamhip
 
 using namespace bamhip;
@@ -135,6 +172,7 @@ int bamhip_init(int device) {
   int n = 0;
   BH_HIP(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return set_error(BAMHIP_EINVAL, "bamhip_init: device %d of %d", device, n);
+  if (g_device >= 0 && g_device != device) array_pool_release();  // recycled arrays live on the old device
   BH_HIP(hipSetDevice(device));
   int e;
   if ((e = noalloc_scratch_init(device))) return e;
@@ -166,6 +204,7 @@ int bamhip_finalize(void) {
     BH_HIP(hipStreamDestroy(g_stream));
     g_stream = nullptr;
   }
+  array_pool_release();
   g_device = -1;
   return BAMHIP_OK;
 }
@@ -294,8 +333,7 @@ int bamhip_level_create(bamhip_level **lev, int l, int nboxes, const bamhip_box 
 int bamhip_level_free(bamhip_level *lev) {
   if (!lev) return BAMHIP_OK;
   if (g_stream) (void)hipStreamSynchronize(g_stream);
-  for (double *p : lev->v)
-    if (p) (void)hipFree(p);
+  for (double *p : lev->v) array_put(p, sizeof(double) * lev->npoints);
   if (lev->d_flag) (void)hipFree(lev->d_flag);
   delete lev;
   return BAMHIP_OK;
@@ -311,9 +349,13 @@ int bamhip_level_enable(bamhip_level *lev, int var) {
   if (!lev || var < 0 || var >= lev->nvariables)
     return set_error(BAMHIP_EINVAL, "bamhip_level_enable: bad variable index %d", var);
   if (lev->v[var]) return BAMHIP_OK;
-  double *p = nullptr;
-  if (hipMalloc(&p, sizeof(double) * lev->npoints) != hipSuccess)
-    return set_error(BAMHIP_ENOMEM, "bamhip_level_enable: out of device memory (%ld points)", lev->npoints);
+  double *p = array_get(sizeof(double) * lev->npoints);
+  if (!p && hipMalloc(&p, sizeof(double) * lev->npoints) != hipSuccess) {
+    // return the pooled arrays to the device and try once more
+    array_pool_release();
+    if (hipMalloc(&p, sizeof(double) * lev->npoints) != hipSuccess)
+      return set_error(BAMHIP_ENOMEM, "bamhip_level_enable: out of device memory (%ld points)", lev->npoints);
+  }
   lev->v[var] = p;
   return BAMHIP_OK;
 }
@@ -323,7 +365,7 @@ int bamhip_level_disable(bamhip_level *lev, int var) {
     return set_error(BAMHIP_EINVAL, "bamhip_level_disable: bad variable index %d", var);
   if (lev->v[var]) {
     BH_HIP(hipStreamSynchronize(g_stream));
-    BH_HIP(hipFree(lev->v[var]));
+    array_put(lev->v[var], sizeof(double) * lev->npoints);
     lev->v[var] = nullptr;
   }
   return BAMHIP_OK;

@@ -286,7 +286,12 @@ int bamhip_p2c_batch(const bamhip_c2p_params *par, long n, const double *prim5, 
                      double *cons);
 
 /* Replaces MATTER.c2p(ucur, primVars) (src/projects/matter/bam_matter.h:49,
- * called at matter.c:317,411) on a level: cons5 / metric6 / prim8 are the
+ * called at matter.c:317,411) on a level.  BAMHIP_C2P_BRACKETED here takes
+ * the hybrid's control flow: the bracketed solve from the previous pressure
+ * replaces try_hot where D >= fatm rhoatm and it returns OK above the
+ * atmosphere threshold with eps >= 0 and v^2 <= vmax; every other point runs
+ * the reference hybrid (cold branch, atmosphere, conserved rewrites).
+ * cons5 / metric6 / prim8 are the
  * variable indices of the conserved variables, of adm_gxx.. and of the
  * primVars list (rho, epsl, vx, vy, vz, p, v2, detg; grhd.c:46-53).
  * status / iters: device arrays of npoints ints. */

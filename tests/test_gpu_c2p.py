@@ -174,3 +174,55 @@ def test_c2p_vanal_matches_oracle(metric_noise):
     assert np.all(np.abs(gp[0] - cp[0])[ok] <= rbar[ok])
     assert np.all(np.abs(gp[5] - cp[5])[ok] <= pbar[ok] + 64 * eps * np.abs(cp[5][ok]))
     print("vanal: %d of %d points OK on both sides, %d status flips" % (ok.sum(), n, flip.sum()), flush=True)
+
+
+def test_level_bracketed_follows_hybrid_control_flow():
+    """MATTER.c2p with the bracketed solver on a level: the reference hybrid's
+    cold branch, atmosphere and conserved rewrites (bitwise where the state
+    sits below the atmosphere density or has negative tau), the same root as
+    the reference where its Newton iteration converges, and a converged root
+    where the reference stalls at its rounding floor"""
+    n = 1 << 15
+    prim5, g, W = sample_prims(n, seed=53)
+    Po = O.c2p_params(errormax=1e-12, countmax=100, rhoatm=1e-12)
+    cons = O.p2c(Po, prim5, g)
+    cons[:, :256] *= 1e-6            # atmosphere-level densities
+    cons[1, 256:512] *= -2.0         # negative tau: the cold branch
+    pprev = (Po.gamma - 1) * prim5[0] * prim5[1] * 1.3
+    cp, cs, ci, cc = O.c2p(Po, cons, g, pprev, rhoguess=prim5[0], method="hybrid")
+    lev = B.Level([B.make_box(n, 1, 1, 0.0, 0.0, 0.0, 1.0)], 20)
+    lev.enable(range(20))
+    for k in range(5):
+        lev.put(k, cons[k])
+    for k in range(6):
+        lev.put(5 + k, g[k])
+    prim0 = np.zeros((8, n))
+    prim0[0], prim0[5] = prim5[0], pprev
+    for k in range(8):
+        lev.put(11 + k, prim0[k])
+    ds, di = B.DeviceArray((n,), np.int32), B.DeviceArray((n,), np.int32)
+    Pg = B.c2p_params(errormax=1e-12, countmax=100, rhoatm=1e-12)
+    lev.grhd_c2p(Pg, B.C2P_BRACKETED, list(range(5)), list(range(5, 11)), list(range(11, 19)), ds, di)
+    st = ds.get()
+    gp = np.array([lev.get(11 + k) for k in range(8)])
+    gc = np.array([lev.get(k) for k in range(5)])
+    lev.free()
+    ds.free()
+    di.free()
+    low = (cons[0] / np.sqrt(np.abs(g[0] * (g[3] * g[5] - g[4] ** 2) - g[1] * (g[1] * g[5] - g[2] * g[4])
+                                    + g[2] * (g[1] * g[4] - g[2] * g[3])))) < Po.fatm * Po.rhoatm
+    # below the atmosphere density: exactly the reference hybrid
+    np.testing.assert_array_equal(st[low], cs[low])
+    np.testing.assert_array_equal(gp[:, low], cp[:, low])
+    np.testing.assert_array_equal(gc[:, low], cc[:, low])
+    # Newton failures only where the reference fails too (a handful); where
+    # the reference converged, the same status
+    assert np.all(cs[st == B.C2P_NR] == O.C2P_NR) and (st == B.C2P_NR).sum() <= 1e-3 * n
+    okc = cs == O.C2P_OK
+    assert np.all(st[okc] == O.C2P_OK)
+    kappa = W ** 2 * (cons[1] + cons[0]) / np.maximum(cp[5], 1e-300)
+    bound = np.maximum(64 * np.finfo(float).eps * kappa, 1e-12)
+    assert np.all(np.abs(gp[5] / cp[5] - 1)[okc] <= bound[okc])
+    # where the reference stalled (NR -> atmosphere), the bracketed root is a root
+    stalled = cs == O.C2P_NR
+    assert stalled.sum() > 0 and np.mean(st[stalled] == O.C2P_OK) > 0.99
