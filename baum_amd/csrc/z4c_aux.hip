@@ -66,12 +66,12 @@ struct RadInfo {
 
 // one thread per surface point of one box, all 25 variables
 // rk = 0: unew = RHS (c == 0) or upre + c RHS at the boundary points; rk = 1
-// (an RK4 stage with the update fused in): uc += wc RHS and, when has_v,
-// unew = upre + wv RHS
+// (an RK4 stage with the update fused in): uc = ucin + wc RHS, when has_v
+// unew = upre + wv RHS, when has_cp copy = upre
 struct RkFuse {
-  Ptrs25 uc;
+  Ptrs25 uc, ucin, copy;
   double wc, wv;
-  int rk, has_v;
+  int rk, has_v, has_cp;
 };
 __global__ void radiative_kernel(Ptrs25 unew, Ptrs25 upre, Ptrs25 ucur, RadInfo ri, DevBox B, double c, RkFuse F) {
   const int M = B.m, N = B.n, O = B.o;
@@ -141,18 +141,26 @@ __global__ void radiative_kernel(Ptrs25 unew, Ptrs25 upre, Ptrs25 ucur, RadInfo 
     rhs -= vx * del(u, vx, Mcc, mcc, pcc, Pcc) / (2 * B.dx) + vy * del(u, vy, cMc, cmc, cpc, cPc) / (2 * B.dy) +
            vz * del(u, vz, ccM, ccm, ccp, ccP) / (2 * B.dz);
     if (F.rk) {
-      F.uc.p[v][q] = F.uc.p[v][q] + F.wc * rhs;
-      if (F.has_v) unew.p[v][q] = upre.p[v][q] + F.wv * rhs;
+      // both loads before the stores: an output may alias ucin or upre
+      const double a = F.ucin.p[v][q], b = (F.has_v || F.has_cp) ? upre.p[v][q] : 0.0;
+      F.uc.p[v][q] = a + F.wc * rhs;
+      if (F.has_v) unew.p[v][q] = b + F.wv * rhs;
+      if (F.has_cp) F.copy.p[v][q] = b;
     } else {
       unew.p[v][q] = (c != 0.0) ? upre.p[v][q] + c * rhs : rhs;
     }
   }
 }
 
-// dst = src for all 25 variables at the points of one box that no RHS kernel
-// computes (the skip layers of width skip[f] at face f), enumerated as x slabs
-// (full yz planes), y slabs (interior x) and z slabs (interior x and y)
-__global__ void skip_copy_kernel(Ptrs25 dst, Ptrs25 src, DevBox B) {
+// dst[0 .. nd) = src for all 25 variables at the points of one box that no
+// RHS kernel computes (the skip layers of width skip[f] at face f),
+// enumerated as x slabs (full yz planes), y slabs (interior x) and z slabs
+// (interior x and y)
+struct SkipDst {
+  Ptrs25 d[3];
+  int nd;
+};
+__global__ void skip_copy_kernel(SkipDst D, Ptrs25 src, DevBox B) {
   const int M = B.m, N = B.n, O = B.o;
   const int *s = B.skip;
   const int ix = M - s[0] - s[1], iy = N - s[2] - s[3];
@@ -183,7 +191,10 @@ __global__ void skip_copy_kernel(Ptrs25 dst, Ptrs25 src, DevBox B) {
   }
   const long q = B.noffset + i + (long)M * j + (long)M * N * k;
 #pragma unroll 5
-  for (int v = 0; v < 25; v++) dst.p[v][q] = src.p[v][q];
+  for (int v = 0; v < 25; v++) {
+    const double x = src.p[v][q];
+    for (int t = 0; t < D.nd; t++) D.d[t].p[v][q] = x;
+  }
 }
 
 // periodic copy of g layers along direction d for nv variables
@@ -640,22 +651,43 @@ int bamhip_z4c_set_boundary(bamhip_level *lev, const bamhip_z4c_params *par, con
   return boundary_impl(lev, par, N, Pp, U, c, F, &N, 1, pgh, "bamhip_z4c_set_boundary");
 }
 
-int bamhip_z4c_bc_rk4_stage(bamhip_level *lev, const bamhip_z4c_params *par, const int *src, const int *uc,
-                            const int *up, const int *vout, double wc, double wv, int pgh) {
-  if (!lev || !par || !src || !uc || !up) return set_error(BAMHIP_EINVAL, "bamhip_z4c_bc_rk4_stage: NULL argument");
+int bamhip_z4c_bc_rk4_stage(bamhip_level *lev, const bamhip_z4c_params *par, const int *src, const int *acc,
+                            const int *accout, const int *base, const int *vout, const int *copy, double wc,
+                            double wv, int pgh, int fill_skip) {
+  if (!lev || !par || !src || !acc || !accout || !base)
+    return set_error(BAMHIP_EINVAL, "bamhip_z4c_bc_rk4_stage: NULL argument");
   Ptrs25 U, Pp, V;
   RkFuse F{};
   int e;
   if ((e = get25(lev, src, 25, U, "bamhip_z4c_bc_rk4_stage(src)"))) return e;
-  if ((e = get25(lev, uc, 25, F.uc, "bamhip_z4c_bc_rk4_stage(uc)"))) return e;
-  if ((e = get25(lev, up, 25, Pp, "bamhip_z4c_bc_rk4_stage(up)"))) return e;
+  if ((e = get25(lev, acc, 25, F.ucin, "bamhip_z4c_bc_rk4_stage(acc)"))) return e;
+  if ((e = get25(lev, accout, 25, F.uc, "bamhip_z4c_bc_rk4_stage(accout)"))) return e;
+  if ((e = get25(lev, base, 25, Pp, "bamhip_z4c_bc_rk4_stage(base)"))) return e;
+  if (copy && (e = get25(lev, copy, 25, F.copy, "bamhip_z4c_bc_rk4_stage(copy)"))) return e;
   F.rk = 1;
   F.wc = wc;
   F.wv = wv;
   F.has_v = vout != nullptr;
+  F.has_cp = copy != nullptr;
   if (vout) {
     if ((e = get25(lev, vout, 25, V, "bamhip_z4c_bc_rk4_stage(vout)"))) return e;
-    // points no kernel computes: v = u_p + wv * 0
+  } else {
+    V = Pp;
+  }
+  // Points no kernel computes (RHS 0 there): the next state and the copy are
+  // base, and an accumulator written to another list than it is read from is
+  // acc + wc * 0 = base there, since acc equals base at those points in the
+  // stage that starts the rotation.  Nothing else writes those points during
+  // the stages, so the caller may fill them in the first stage only.
+  SkipDst D{};
+  if (vout) D.d[D.nd++] = V;
+  if (copy) D.d[D.nd++] = F.copy;
+  if (accout != acc) {
+    bool same = true;
+    for (int v = 0; v < 25; v++) same = same && F.uc.p[v] == F.ucin.p[v];
+    if (!same) D.d[D.nd++] = F.uc;
+  }
+  if (fill_skip && D.nd > 0) {
     for (const DevBox &B : lev->boxes) {
       const long ix = B.m - B.skip[0] - B.skip[1], iy = B.n - B.skip[2] - B.skip[3];
       if (ix < 1 || iy < 1 || B.o - B.skip[4] - B.skip[5] < 1)
@@ -663,11 +695,9 @@ int bamhip_z4c_bc_rk4_stage(bamhip_level *lev, const bamhip_z4c_params *par, con
       const long total = (long)(B.skip[0] + B.skip[1]) * B.n * B.o + ix * (B.skip[2] + B.skip[3]) * B.o +
                          ix * iy * (B.skip[4] + B.skip[5]);
       if (total <= 0) continue;
-      skip_copy_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream()>>>(V, Pp, B);
+      skip_copy_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream()>>>(D, Pp, B);
       BH_HIP(hipGetLastError());
     }
-  } else {
-    V = Pp;
   }
   const Ptrs25 lists[2] = {F.uc, V};
   return boundary_impl(lev, par, V, Pp, U, 0.0, F, lists, vout ? 2 : 1, pgh, "bamhip_z4c_bc_rk4_stage");

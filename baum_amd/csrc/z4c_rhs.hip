@@ -56,13 +56,17 @@ __device__ __forceinline__ void static_for(F &&f) {
 namespace bamhip {
 
 // u: the state the stencils read; r: output (the RHS, unew = upre + c RHS, or
-// in an RK4 stage the next stage's state); p: upre, or u_p in an RK4 stage;
-// c: the RK4 accumulator u_c (read and written at the point only)
+// in an RK4 stage the next stage's state); p: upre, or the RK4 step's start
+// state in a stage.  In an RK4 stage (read and written at the point only):
+// a: the accumulator read, c: the accumulator written (c = a + wc RHS), cp:
+// where the start state p is copied to (or unused).
 struct Z4cPtrs {
   const double *u[BAMHIP_Z4C_NVARS];
   double *r[BAMHIP_Z4C_NVARS];
   const double *p[BAMHIP_Z4C_NVARS];
   double *c[BAMHIP_Z4C_NVARS];
+  const double *a[BAMHIP_Z4C_NVARS];
+  double *cp[BAMHIP_Z4C_NVARS];
 };
 
 struct Z4cGeo {
@@ -86,9 +90,9 @@ struct Z4cK {
   double chiDivFloor, chipsipower;
   double c;
   int addlinear;
-  // RK4 stage (rk.c:231-252) fused into the store: u_c += wc RHS and, when
-  // has_v, v_next = u_p + wv RHS
-  int rk, has_v;
+  // RK4 stage (rk.c:231-252) fused into the store: c = a + wc RHS, when
+  // has_v r = p + wv RHS, when has_cp cp = p
+  int rk, has_v, has_cp;
   double wc, wv;
 };
 
@@ -209,20 +213,25 @@ struct Edge {
 // normalisation (z4_rhs_movpunc_N.m:446-467)
 __device__ __forceinline__ void store_point(const Z4cPtrs &P, const Z4cK &K, long q, double *rhs) {
   if (K.rk) {
-    // all loads before any store: the lists may alias as far as the compiler
-    // knows, so loads written after a store would each wait for it
-    double cv[25], pv[25];
+    // all loads before any store: the lists may alias (the accumulator read
+    // and the next state written at the point, for one), and loads written
+    // after a store would each wait for it
+    double av[25], pv[25];
 #pragma unroll
-    for (int v = 0; v < 25; v++) cv[v] = P.c[v][q];
-    if (K.has_v) {
+    for (int v = 0; v < 25; v++) av[v] = P.a[v][q];
+    if (K.has_v || K.has_cp) {
 #pragma unroll
       for (int v = 0; v < 25; v++) pv[v] = P.p[v][q];
     }
 #pragma unroll
-    for (int v = 0; v < 25; v++) P.c[v][q] = cv[v] + K.wc * rhs[v];
+    for (int v = 0; v < 25; v++) P.c[v][q] = av[v] + K.wc * rhs[v];
     if (K.has_v) {
 #pragma unroll
       for (int v = 0; v < 25; v++) P.r[v][q] = pv[v] + K.wv * rhs[v];
+    }
+    if (K.has_cp) {
+#pragma unroll
+      for (int v = 0; v < 25; v++) P.cp[v][q] = pv[v];
     }
     return;
   }
@@ -903,6 +912,8 @@ struct TileJobs {
   Spacing sp;
 };
 static_assert(sizeof(TileJobs) <= 3072, "kernel argument size");
+static_assert(sizeof(Z4cPtrs) + sizeof(TileJobs) + sizeof(Z4cK) + sizeof(int *) <= 4096,
+              "arguments of the tile kernel within 4 KiB");
 
 // SH = false: every point of the regions is >= 3 from each edge (standard
 // stencils).  SH = true: the regions are slabs of the box-edge shell; only
@@ -1426,6 +1437,7 @@ static Z4cK make_k(const bamhip_z4c_params *p, double c) {
   K.addlinear = (c != 0.0);
   K.rk = 0;
   K.has_v = 0;
+  K.has_cp = 0;
   K.wc = K.wv = 0.0;
   return K;
 }
@@ -1838,7 +1850,11 @@ int z4c_rhs_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int *une
   } else {
     for (int v = 0; v < 25; v++) P.p[v] = P.u[v];
   }
-  for (int v = 0; v < 25; v++) P.c[v] = nullptr;
+  for (int v = 0; v < 25; v++) {
+    P.c[v] = nullptr;
+    P.a[v] = nullptr;
+    P.cp[v] = nullptr;
+  }
   for (int v = 0; v < 25; v++)
     for (int w = 0; w < 25; w++)
       if (P.r[v] == P.u[w])
@@ -1865,28 +1881,47 @@ static int check_params(const bamhip_z4c_params *par, const char *what) {
 }
 
 // One RK4 stage with the update fused into the RHS kernels' store: at every
-// computed point u_c += wc RHS(src) and (vout != NULL) vout = u_p + wv RHS(src).
-// src is read by the stencils, so neither u_c nor vout may alias it; u_c is
-// read and written at the point only.  scratch (optional): 25 variables the
-// tuner may use as a plain RHS output the first time a box set is seen.
-int z4c_rhs_rk4_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int *src, const int *uc, const int *up,
-                     const int *vout, double wc, double wv, const int *scratch) {
-  if (!lev || !par || !src || !uc || !up) return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs_rk4_stage: NULL argument");
+// computed point accout = acc + wc RHS(src), when vout != NULL vout = base +
+// wv RHS(src), and when copy != NULL copy = base.  src is read by the
+// stencils, so no output may alias it; acc and base are read at the point
+// only, before any store, so an output may alias them (the accumulator read
+// from the list the next state goes to, for one).  The outputs must differ
+// from each other.  scratch (optional): 25 variables the tuner may use as a
+// plain RHS output the first time a box set is seen; it must not alias an
+// input (src, acc, base).
+int z4c_rhs_rk4_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int *src, const int *acc,
+                     const int *accout, const int *base, const int *vout, const int *copy, double wc, double wv,
+                     const int *scratch) {
+  if (!lev || !par || !src || !acc || !accout || !base)
+    return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs_rk4_stage: NULL argument");
   int e;
   if ((e = check_params(par, "bamhip_z4c_rhs_rk4_stage"))) return e;
   Z4cPtrs P;
   if ((e = gather_ptrs(lev, src, 25, const_cast<double **>(P.u), "bamhip_z4c_rhs_rk4_stage(src)"))) return e;
-  if ((e = gather_ptrs(lev, uc, 25, P.c, "bamhip_z4c_rhs_rk4_stage(uc)"))) return e;
-  if ((e = gather_ptrs(lev, up, 25, const_cast<double **>(P.p), "bamhip_z4c_rhs_rk4_stage(up)"))) return e;
+  if ((e = gather_ptrs(lev, acc, 25, const_cast<double **>(P.a), "bamhip_z4c_rhs_rk4_stage(acc)"))) return e;
+  if ((e = gather_ptrs(lev, accout, 25, P.c, "bamhip_z4c_rhs_rk4_stage(accout)"))) return e;
+  if ((e = gather_ptrs(lev, base, 25, const_cast<double **>(P.p), "bamhip_z4c_rhs_rk4_stage(base)"))) return e;
   if (vout) {
     if ((e = gather_ptrs(lev, vout, 25, P.r, "bamhip_z4c_rhs_rk4_stage(vout)"))) return e;
   } else {
     for (int v = 0; v < 25; v++) P.r[v] = nullptr;
   }
+  if (copy) {
+    if ((e = gather_ptrs(lev, copy, 25, P.cp, "bamhip_z4c_rhs_rk4_stage(copy)"))) return e;
+  } else {
+    for (int v = 0; v < 25; v++) P.cp[v] = nullptr;
+  }
   for (int v = 0; v < 25; v++)
     for (int w = 0; w < 25; w++) {
-      if (P.c[v] == P.u[w] || (vout && (P.r[v] == P.u[w] || P.r[v] == P.c[w] || P.r[v] == P.p[w])))
-        return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs_rk4_stage: uc / vout alias src, up or each other");
+      const double *outs[3] = {P.c[v], P.r[v], P.cp[v]}, *outw[3] = {P.c[w], P.r[w], P.cp[w]};
+      for (int x = 0; x < 3; x++) {
+        if (!outs[x]) continue;
+        if (outs[x] == P.u[w])
+          return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs_rk4_stage: an output list aliases src");
+        for (int y = 0; y < 3; y++)
+          if (y != x && outs[x] == outw[y])
+            return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs_rk4_stage: the output lists alias each other");
+      }
     }
   for (const DevBox &B : lev->boxes)
     for (int f = 0; f < 6; f++)
@@ -1897,12 +1932,13 @@ int z4c_rhs_rk4_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int 
     if ((e = gather_ptrs(lev, scratch, 25, scr, "bamhip_z4c_rhs_rk4_stage(scratch)"))) return e;
     for (int v = 0; v < 25; v++)
       for (int w = 0; w < 25; w++)
-        if (scr[v] == P.u[w] || scr[v] == P.c[w] || scr[v] == P.p[w] || (vout && scr[v] == P.r[w]))
-          return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs_rk4_stage: scratch aliases another list");
+        if (scr[v] == P.u[w] || scr[v] == P.a[w] || scr[v] == P.p[w])
+          return set_error(BAMHIP_EINVAL, "bamhip_z4c_rhs_rk4_stage: scratch aliases an input list");
   }
   Z4cK K = make_k(par, 0.0);
   K.rk = 1;
   K.has_v = vout ? 1 : 0;
+  K.has_cp = copy ? 1 : 0;
   K.wc = wc;
   K.wv = wv;
   return run_rhs(lev, P, K, lev->check_mode != 0, scratch ? scr : nullptr);
@@ -1911,9 +1947,9 @@ int z4c_rhs_rk4_impl(bamhip_level *lev, const bamhip_z4c_params *par, const int 
 }  // namespace bamhip
 
 extern "C" int bamhip_z4c_rhs_rk4_stage(bamhip_level *lev, const bamhip_z4c_params *par, const int *src,
-                                        const int *uc, const int *up, const int *vout, double wc, double wv,
-                                        const int *scratch) {
-  return bamhip::z4c_rhs_rk4_impl(lev, par, src, uc, up, vout, wc, wv, scratch);
+                                        const int *acc, const int *accout, const int *base, const int *vout,
+                                        const int *copy, double wc, double wv, const int *scratch) {
+  return bamhip::z4c_rhs_rk4_impl(lev, par, src, acc, accout, base, vout, copy, wc, wv, scratch);
 }
 
 extern "C" int bamhip_z4c_rhs(bamhip_level *lev, const bamhip_z4c_params *par, const int *unew, const int *upre,

@@ -33,24 +33,37 @@ int bamhip_z4c_rk4_stages(bamhip_level *lev, const bamhip_z4c_params *par, doubl
   const double wc[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
   const double wv[4] = {dt / 2, dt / 2, dt, 0.0};
   int e, s;
-  /* u_p = u_c */
-  if ((e = bamhip_vladd(lev, BAMHIP_Z4C_NVARS, up, 1.0, uc, 0.0, 0))) return e;
   long npoints = 0;
   if ((e = bamhip_level_npoints(lev, &npoints))) return e;
   if (g_fused == 1 || (g_fused == 2 && npoints >= BAMHIP_FUSE_MIN_POINTS)) {
-    /* The update inside the RHS kernels' store.  u_c is written at each
-     * point while the stencils read the stage state, so stage 1 reads u_p
-     * (equal to u_c at this point) and the stage states alternate between v
-     * and r: stage s reads src[s] and writes dst[s]. */
-    const int *src[4] = {up, v, r, v};
-    const int *dst[4] = {v, r, v, 0};
+    /* The update inside the RHS kernels' store, the four lists rotating so
+     * that no stage writes a list its stencils read and no pass copies
+     * u_p = u_c (U = uc, P = up):
+     *   stage 1 reads U:   r = U + wc k, v = U + wv k, P = U
+     *   stage 2 reads v:   U = r + wc k, r = P + wv k
+     *   stage 3 reads r:   U = U + wc k, v = P + wv k
+     *   stage 4 reads v:   U = U + wc k
+     * The same sums as the separate passes, so bitwise the same state; P
+     * ends as the step's start state.  Stage 1 also sets r, v and P to U at
+     * the points no kernel computes, which no later stage writes. */
+    const int *src[4] = {uc, v, r, v};
+    const int *acc[4] = {uc, r, uc, uc};
+    const int *accout[4] = {r, uc, uc, uc};
+    const int *base[4] = {uc, up, up, up};
+    const int *vout[4] = {v, r, v, 0};
     for (s = 0; s < 4; s++) {
-      if ((e = bamhip_z4c_rhs_rk4_stage(lev, par, src[s], uc, up, dst[s], wc[s], wv[s], s == 0 ? r : 0)))
+      const int *copy = s == 0 ? up : 0;
+      if ((e = bamhip_z4c_rhs_rk4_stage(lev, par, src[s], acc[s], accout[s], base[s], vout[s], copy, wc[s], wv[s],
+                                        s == 0 ? r : 0)))
         return e;
-      if ((e = bamhip_z4c_bc_rk4_stage(lev, par, src[s], uc, up, dst[s], wc[s], wv[s], pgh))) return e;
+      if ((e = bamhip_z4c_bc_rk4_stage(lev, par, src[s], acc[s], accout[s], base[s], vout[s], copy, wc[s], wv[s], pgh,
+                                       s == 0)))
+        return e;
     }
     return BAMHIP_OK;
   }
+  /* u_p = u_c */
+  if ((e = bamhip_vladd(lev, BAMHIP_Z4C_NVARS, up, 1.0, uc, 0.0, 0))) return e;
   for (s = 0; s < 4; s++) {
     const int *src = (s == 0) ? uc : v;
     if ((e = bamhip_z4c_rhs(lev, par, r, 0, 0.0, src))) return e;

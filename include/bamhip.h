@@ -541,23 +541,28 @@ int bamhip_rk4_last_update(bamhip_level *lev, const int *uc, const int *r, doubl
                            int algcon, int subtractA, int normalizedetg);
 
 /* One RK4 stage of evolve_rk_rk4 (rk.c:231-252: evolve_rhs, set_boundary,
- * vladdto, vladd) with the update fused into the RHS kernels: at every point
- * uc += wc * RHS(src) and, when vout != NULL, vout = up + wv * RHS(src); the
- * boundary call does the same on PHYBOUND faces (radiative RHS), sets
- * vout = up at the other points no RHS kernel computes, and copies the
- * periodic layers of uc and vout.  src must alias neither uc nor vout.
- * scratch (optional, 25 indices): a list the tile-kernel tuner may overwrite
- * with a plain RHS the first time a set of boxes is seen.  With fusion
- * selected, bamhip_z4c_rk4_stages runs stage 1 from up (= uc) and alternates
- * the stage states between v and r; results are bitwise those of the unfused
- * stages when the periodic layers of uc start consistent.
+ * vladdto, vladd) with the update fused into the RHS kernels: at every
+ * computed point accout = acc + wc * RHS(src), when vout != NULL
+ * vout = base + wv * RHS(src), and when copy != NULL copy = base.  The
+ * boundary call does the same on PHYBOUND faces (radiative RHS), copies the
+ * periodic layers of accout and vout, and with fill_skip sets vout, copy and
+ * (when it is another list than acc) accout to base at the other points no
+ * RHS kernel computes.  No output may alias src or another output; acc and
+ * base are read at the point only and may alias outputs.  scratch
+ * (optional, 25 indices, not an input): a list the tile-kernel tuner may
+ * overwrite with a plain RHS the first time a set of boxes is seen.  With
+ * fusion selected, bamhip_z4c_rk4_stages rotates uc, up, r and v through
+ * these roles so that no pass copies up = uc (its comment lists the roles);
+ * results are bitwise those of the unfused stages when the periodic layers
+ * of uc start consistent.
  * bamhip_z4c_set_rk_fusion(mode): 0 never, 1 always, 2 (default) on levels of
  * at least 2^22 points; returns the previous mode. */
 int bamhip_z4c_rhs_rk4_stage(bamhip_level *lev, const bamhip_z4c_params *par, const int *src,
-                             const int *uc, const int *up, const int *vout, double wc, double wv,
-                             const int *scratch);
+                             const int *acc, const int *accout, const int *base, const int *vout,
+                             const int *copy, double wc, double wv, const int *scratch);
 int bamhip_z4c_bc_rk4_stage(bamhip_level *lev, const bamhip_z4c_params *par, const int *src,
-                            const int *uc, const int *up, const int *vout, double wc, double wv, int pgh);
+                            const int *acc, const int *accout, const int *base, const int *vout,
+                            const int *copy, double wc, double wv, int pgh, int fill_skip);
 int bamhip_z4c_set_rk_fusion(int mode);
 
 /* -------------------------------------------------------- GRHD fluxes */
