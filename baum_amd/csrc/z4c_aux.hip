@@ -152,13 +152,14 @@ __global__ void radiative_kernel(Ptrs25 unew, Ptrs25 upre, Ptrs25 ucur, RadInfo 
   }
 }
 
-// dst[0 .. nd) = src for all 25 variables at the points of one box that no
-// RHS kernel computes (the skip layers of width skip[f] at face f),
-// enumerated as x slabs (full yz planes), y slabs (interior x) and z slabs
-// (interior x and y)
+// dst[0 .. nd) = src (or the constant val when fill is set) for nv <= 25
+// variables at the points of one box that no RHS kernel computes (the skip
+// layers of width skip[f] at face f), enumerated as x slabs (full yz
+// planes), y slabs (interior x) and z slabs (interior x and y)
 struct SkipDst {
   Ptrs25 d[3];
-  int nd;
+  int nd, nv, fill;
+  double val;
 };
 __global__ void skip_copy_kernel(SkipDst D, Ptrs25 src, DevBox B) {
   const int M = B.m, N = B.n, O = B.o;
@@ -190,11 +191,25 @@ __global__ void skip_copy_kernel(SkipDst D, Ptrs25 src, DevBox B) {
     return;
   }
   const long q = B.noffset + i + (long)M * j + (long)M * N * k;
-#pragma unroll 5
-  for (int v = 0; v < 25; v++) {
-    const double x = src.p[v][q];
+  for (int v = 0; v < D.nv; v++) {
+    const double x = D.fill ? D.val : src.p[v][q];
     for (int t = 0; t < D.nd; t++) D.d[t].p[v][q] = x;
   }
+}
+
+// launch skip_copy_kernel over every box of the level
+static int skip_launch(bamhip_level *lev, const SkipDst &D, const Ptrs25 &src, const char *what) {
+  for (const DevBox &B : lev->boxes) {
+    const long ix = B.m - B.skip[0] - B.skip[1], iy = B.n - B.skip[2] - B.skip[3];
+    if (ix < 1 || iy < 1 || B.o - B.skip[4] - B.skip[5] < 1)
+      return set_error(BAMHIP_EINVAL, "%s: a box has no inner points", what);
+    const long total = (long)(B.skip[0] + B.skip[1]) * B.n * B.o + ix * (B.skip[2] + B.skip[3]) * B.o +
+                       ix * iy * (B.skip[4] + B.skip[5]);
+    if (total <= 0) continue;
+    skip_copy_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream()>>>(D, src, B);
+    BH_HIP(hipGetLastError());
+  }
+  return BAMHIP_OK;
 }
 
 // periodic copy of g layers along direction d for nv variables
@@ -680,6 +695,7 @@ int bamhip_z4c_bc_rk4_stage(bamhip_level *lev, const bamhip_z4c_params *par, con
   // stage that starts the rotation.  Nothing else writes those points during
   // the stages, so the caller may fill them in the first stage only.
   SkipDst D{};
+  D.nv = 25;
   if (vout) D.d[D.nd++] = V;
   if (copy) D.d[D.nd++] = F.copy;
   if (accout != acc) {
@@ -687,20 +703,21 @@ int bamhip_z4c_bc_rk4_stage(bamhip_level *lev, const bamhip_z4c_params *par, con
     for (int v = 0; v < 25; v++) same = same && F.uc.p[v] == F.ucin.p[v];
     if (!same) D.d[D.nd++] = F.uc;
   }
-  if (fill_skip && D.nd > 0) {
-    for (const DevBox &B : lev->boxes) {
-      const long ix = B.m - B.skip[0] - B.skip[1], iy = B.n - B.skip[2] - B.skip[3];
-      if (ix < 1 || iy < 1 || B.o - B.skip[4] - B.skip[5] < 1)
-        return set_error(BAMHIP_EINVAL, "bamhip_z4c_bc_rk4_stage: a box has no inner points");
-      const long total = (long)(B.skip[0] + B.skip[1]) * B.n * B.o + ix * (B.skip[2] + B.skip[3]) * B.o +
-                         ix * iy * (B.skip[4] + B.skip[5]);
-      if (total <= 0) continue;
-      skip_copy_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream()>>>(D, Pp, B);
-      BH_HIP(hipGetLastError());
-    }
-  }
+  if (fill_skip && D.nd > 0 && (e = skip_launch(lev, D, Pp, "bamhip_z4c_bc_rk4_stage"))) return e;
   const Ptrs25 lists[2] = {F.uc, V};
   return boundary_impl(lev, par, V, Pp, U, 0.0, F, lists, vout ? 2 : 1, pgh, "bamhip_z4c_bc_rk4_stage");
+}
+
+int bamhip_skip_fill(bamhip_level *lev, int n, const int *vars, double value) {
+  if (!lev || !vars || n < 1 || n > 25) return set_error(BAMHIP_EINVAL, "bamhip_skip_fill: bad arguments");
+  SkipDst D{};
+  int e;
+  if ((e = get25(lev, vars, n, D.d[0], "bamhip_skip_fill"))) return e;
+  D.nd = 1;
+  D.nv = n;
+  D.fill = 1;
+  D.val = value;
+  return skip_launch(lev, D, D.d[0], "bamhip_skip_fill");
 }
 
 int bamhip_z4c_algcon(bamhip_level *lev, const int *g, const int *A, int subtractA, int normalizedetg) {

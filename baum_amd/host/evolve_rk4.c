@@ -60,7 +60,9 @@ int bamhip_z4c_rk4_stages(bamhip_level *lev, const bamhip_z4c_params *par, doubl
                                        s == 0)))
         return e;
     }
-    return BAMHIP_OK;
+    /* r held stage states, and u_p at the points no kernel computes; the
+     * separate passes of a later step take r = 0 there */
+    return bamhip_skip_fill(lev, BAMHIP_Z4C_NVARS, r, 0.0);
   }
   /* u_p = u_c */
   if ((e = bamhip_vladd(lev, BAMHIP_Z4C_NVARS, up, 1.0, uc, 0.0, 0))) return e;

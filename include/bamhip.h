@@ -564,6 +564,11 @@ int bamhip_z4c_bc_rk4_stage(bamhip_level *lev, const bamhip_z4c_params *par, con
                             const int *acc, const int *accout, const int *base, const int *vout,
                             const int *copy, double wc, double wv, int pgh, int fill_skip);
 int bamhip_z4c_set_rk_fusion(int mode);
+/* value at the points of every box that no RHS kernel computes (the skip
+ * layers) for the n <= 25 variables vars; no reference counterpart.  The
+ * fused stages end with it on r, whose skip points the separate passes take
+ * as 0 (a list that only ever holds right-hand sides there). */
+int bamhip_skip_fill(bamhip_level *lev, int n, const int *vars, double value);
 
 /* -------------------------------------------------------- GRHD fluxes */
 /* hrsc_flux / hrsc_rec / EOS choices of grhd_flx1d_fvpr (grhd.c:193-209,

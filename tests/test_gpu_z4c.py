@@ -314,6 +314,38 @@ def test_fused_rk4_stages_bitwise_equal_unfused(case):
     np.testing.assert_array_equal(out[1], out[0])
 
 
+def test_fused_then_separate_stages_on_one_level():
+    """two fused RK4 steps, then one with the separate passes on the same
+    level, give bitwise the state of three steps with the separate passes:
+    the fused stages leave the RHS list as the separate passes expect it
+    (zero at the points no kernel computes)"""
+    h = 0.5
+    specs = [(22, 20, 18, (-5.25, -4.7, -4.3)), (16, 18, 14, (6.0, -4.0, -3.0))]
+    boxes, us = [], []
+    for m, nn, o, x0 in specs:
+        ob = O.make_box(m, nn, o, *x0, h, skip=(1,) * 6)
+        us.append(puncture_z4c(grid(ob), masses=((1.0, (0.05, -0.02, 0.03)),)).reshape(25, -1))
+        boxes.append(B.make_box(m, nn, o, *x0, h, skip=(1,) * 6, bflag=(B.REFBOUND,) * 6))
+    state, dt = np.concatenate(us, axis=1), 0.25 * h
+    out = []
+    try:
+        for modes in ((1, 1, 0), (0, 0, 0)):
+            lev = B.Level(boxes, 100)
+            lev.enable(range(100))
+            for v in range(100):
+                lev.fill(v, 0.0)
+            lev.put_state(U, state)
+            for mode in modes:
+                B.set_rk_fusion(mode)
+                lev.z4c_rk4_step(B.z4c_params(), dt, U, P, R, V, pgh=0)
+            out.append(lev.get_state(U))
+            lev.free()
+    finally:
+        B.set_rk_fusion(2)
+    assert np.abs(out[1] - state).max() > 1e-9
+    np.testing.assert_array_equal(out[0], out[1])
+
+
 def test_constraint_norms_tracked_over_time():
     """Hamiltonian and momentum constraint norms over 6 RK4 steps of the
     small puncture box (adm.c:14-74 at every step), device against oracle:
