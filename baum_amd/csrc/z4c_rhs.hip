@@ -596,16 +596,23 @@ __device__ __forceinline__ bool z4c_point(D &dv, const Z4cPtrs &P, const Z4cK &K
 // 2 x 16 x 8, 16 x 2 x 8 and 16 x 8 x 2 on the x, y and z faces.
 constexpr int HALO = 3;
 constexpr int NTHR = 256;
-template <int TX_, int TY_, int TZ_>
+// PX: unused LDS columns at the end of each tile row (the row stride SX is
+// the staged width WX plus PX), chosen so that the 16 lanes of a read phase
+// fall on distinct banks
+template <int TX_, int TY_, int TZ_, int PX = 0>
 struct Lay {
   static constexpr int TX = TX_, TY = TY_, TZ = TZ_;
-  static constexpr int SX = TX + 2 * HALO, SY = TY + 2 * HALO, SZ = TZ + 2 * HALO;
+  static constexpr int WX = TX + 2 * HALO;
+  static constexpr int SX = WX + PX, SY = TY + 2 * HALO, SZ = TZ + 2 * HALO;
   static constexpr int TILE = SX * SY * SZ;
   static constexpr int NLOAD = (TILE + NTHR - 1) / NTHR;
   static_assert(TX * TY * TZ == NTHR, "one point per thread");
 };
 using LBulk = Lay<16, 4, 4>;
-using LSlabX = Lay<2, 16, 8>;
+// x faces: rows of 2 points, so without padding (stride 8 doubles) the
+// lanes (x, y) = (0..1, 0..7) of a read phase meet on 4 banks; a stride of 10
+// spreads them over all 32
+using LSlabX = Lay<2, 16, 8, 2>;
 using LSlabY = Lay<16, 2, 8>;
 using LSlabZ = Lay<16, 8, 2>;
 using LLayX = Lay<1, 16, 16>;
@@ -784,7 +791,7 @@ struct Stager {
       const int idx = tid + k * NTHR;
       const int x = idx % L::SX, y = (idx / L::SX) % L::SY, z = idx / (L::SX * L::SY);
       const int gi = i0 - HALO + x, gj = j0 - HALO + y, gk = k0 - HALO + z;
-      const bool in = idx < L::TILE && gi >= 0 && gj >= 0 && gk >= 0 && gi < G.m && gj < G.n && gk < G.o;
+      const bool in = idx < L::TILE && x < L::WX && gi >= 0 && gj >= 0 && gk >= 0 && gi < G.m && gj < G.n && gk < G.o;
       goff[k] = in ? (int)(x + G.sj * y + G.sk * z) : 0;
       ok |= in ? (1u << k) : 0u;
     }
@@ -837,7 +844,7 @@ struct DmaStager {
       const int e = VEC == 1 ? d >> 1 : 2 * d;
       const int x = e % L::SX, y = (e / L::SX) % L::SY, z = e / (L::SX * L::SY);
       const int gi = i0 - HALO + x, gj = j0 - HALO + y, gk = k0 - HALO + z;
-      const bool in = e < L::TILE && gi >= 0 && gj >= 0 && gk >= 0 && gi < G.m && gj < G.n && gk < G.o;
+      const bool in = e < L::TILE && x < L::WX && gi >= 0 && gj >= 0 && gk >= 0 && gi < G.m && gj < G.n && gk < G.o;
       const long el = in ? cbase + x + G.sj * y + G.sk * z : G.noffset;
       boff[c] = (unsigned)(el * 8 + (VEC == 1 && in ? (d & 1) * 4 : 0));
     }
