@@ -155,30 +155,38 @@ def test_hierarchy_puncture_tracking_matches_oracle():
         lv.free()
 
 
-def test_graph_replay_matches_eager():
-    dt0 = 0.125
-    pa, pb = B.DeviceArray.from_host(PUNC0), B.DeviceArray.from_host(PUNC0)
-    HA, LA = _gpu_hierarchy(dt0, punctures=pa)
-    HB, LB = _gpu_hierarchy(dt0, punctures=pb)
-    HA.advance(0)            # first step: time interpolation of reduced order
-    HB.advance(0)
-    g = HA.capture(0)        # records the second coarse step, executes nothing
+@pytest.mark.parametrize("fusion", [2, 1])
+def test_graph_replay_matches_eager(fusion):
+    """a captured coarse step replayed three times equals the eager steps
+    bitwise; fusion 1 runs every level's RK4 stages inside the RHS kernels
+    (the rotated lists, the skip-point fills) in the graph"""
+    old = B.set_rk_fusion(fusion)
     try:
-        for step in range(3):
-            HA.launch(g)
-            HB.advance(0)
-            # eager work and host synchronisation between replays
-            l2a = LA[0].z4c_ham_l2(U)
-            l2b = LB[0].z4c_ham_l2(U)
-            assert l2a == l2b
-            for la, lb in zip(LA, LB):
-                np.testing.assert_array_equal(la.get_state(U), lb.get_state(U))
-                np.testing.assert_array_equal(la.get_state(UPP), lb.get_state(UPP))
-            np.testing.assert_array_equal(pa.get(), pb.get())
+        dt0 = 0.125
+        pa, pb = B.DeviceArray.from_host(PUNC0), B.DeviceArray.from_host(PUNC0)
+        HA, LA = _gpu_hierarchy(dt0, punctures=pa)
+        HB, LB = _gpu_hierarchy(dt0, punctures=pb)
+        HA.advance(0)            # first step: time interpolation of reduced order
+        HB.advance(0)
+        g = HA.capture(0)        # records the second coarse step, executes nothing
+        try:
+            for step in range(3):
+                HA.launch(g)
+                HB.advance(0)
+                # eager work and host synchronisation between replays
+                l2a = LA[0].z4c_ham_l2(U)
+                l2b = LB[0].z4c_ham_l2(U)
+                assert l2a == l2b
+                for la, lb in zip(LA, LB):
+                    np.testing.assert_array_equal(la.get_state(U), lb.get_state(U))
+                    np.testing.assert_array_equal(la.get_state(UPP), lb.get_state(UPP))
+                np.testing.assert_array_equal(pa.get(), pb.get())
+        finally:
+            B.Hierarchy.free_graph(g)
+        assert HA.nonfinite() == 0
+        assert np.abs(pa.get() - PUNC0).max() > 1e-6
     finally:
-        B.Hierarchy.free_graph(g)
-    assert HA.nonfinite() == 0
-    assert np.abs(pa.get() - PUNC0).max() > 1e-6
+        B.set_rk_fusion(old)
 
 
 @pytest.mark.parametrize("iteration", [0, 1])
