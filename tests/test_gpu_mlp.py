@@ -119,7 +119,7 @@ def test_learned_guess_polish_matches_bracketed_and_saves_iterations():
     st, it, prim = ds.get(), di.get(), dpr.get()
     assert np.all(st == B.C2P_OK)
     assert fb.get()[0] <= 1e-3 * n
-    # same root as the judge, within the inversion's conditioning
+    # same root as the bracketed solve, within the inversion's conditioning
     pb = test["prim"]
     kappa = test["W"] ** 2 * (test["cons0"][1] + test["cons0"][0]) / pb[5]
     bound = np.maximum(64 * np.finfo(float).eps * kappa, 1e-13)
