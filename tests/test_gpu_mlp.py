@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _solved(n, seed, tol=1e-12):
-    """C4 points: device p2c, bracketed solve (the judge), training data"""
+    """C4 points: device p2c, bracketed solve (the root the other solvers are checked against), training data"""
     prim5, g, W = sample_prims(n, seed=seed)
     P = B.c2p_params(errormax=tol, countmax=100, rhoatm=1e-20)
     dp5, dg = B.DeviceArray.from_host(prim5), B.DeviceArray.from_host(g)
