@@ -155,7 +155,9 @@ __global__ void radiative_kernel(Ptrs25 unew, Ptrs25 upre, Ptrs25 ucur, RadInfo 
 // dst[0 .. nd) = src (or the constant val when fill is set) for nv <= 25
 // variables at the points of one box that no RHS kernel computes (the skip
 // layers of width skip[f] at face f), enumerated as x slabs (full yz
-// planes), y slabs (interior x) and z slabs (interior x and y)
+// planes), y slabs (interior x) and z slabs (interior x and y), x fastest in
+// the y and z slabs so that a wave's accesses are contiguous.  One thread
+// per point and variable (blockIdx.y).
 struct SkipDst {
   Ptrs25 d[3];
   int nd, nv, fill;
@@ -176,25 +178,24 @@ __global__ void skip_copy_kernel(SkipDst D, Ptrs25 src, DevBox B) {
     j = (int)(r % N);
     k = (int)(r / N);
   } else if ((r -= cx) < cy) {
+    i = s[0] + (int)(r % ix);
+    r /= ix;
     const int l = (int)(r % (s[2] + s[3]));
-    r /= (s[2] + s[3]);
     j = l < s[2] ? l : N - s[3] + (l - s[2]);
-    i = s[0] + (int)(r % ix);
-    k = (int)(r / ix);
+    k = (int)(r / (s[2] + s[3]));
   } else if ((r -= cy) < cz) {
-    const int l = (int)(r % (s[4] + s[5]));
-    r /= (s[4] + s[5]);
-    k = l < s[4] ? l : O - s[5] + (l - s[4]);
     i = s[0] + (int)(r % ix);
-    j = s[2] + (int)(r / ix);
+    r /= ix;
+    j = s[2] + (int)(r % iy);
+    const int l = (int)(r / iy);
+    k = l < s[4] ? l : O - s[5] + (l - s[4]);
   } else {
     return;
   }
   const long q = B.noffset + i + (long)M * j + (long)M * N * k;
-  for (int v = 0; v < D.nv; v++) {
-    const double x = D.fill ? D.val : src.p[v][q];
-    for (int t = 0; t < D.nd; t++) D.d[t].p[v][q] = x;
-  }
+  const int v = blockIdx.y;
+  const double x = D.fill ? D.val : src.p[v][q];
+  for (int t = 0; t < D.nd; t++) D.d[t].p[v][q] = x;
 }
 
 // launch skip_copy_kernel over every box of the level
@@ -206,7 +207,7 @@ static int skip_launch(bamhip_level *lev, const SkipDst &D, const Ptrs25 &src, c
     const long total = (long)(B.skip[0] + B.skip[1]) * B.n * B.o + ix * (B.skip[2] + B.skip[3]) * B.o +
                        ix * iy * (B.skip[4] + B.skip[5]);
     if (total <= 0) continue;
-    skip_copy_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream()>>>(D, src, B);
+    skip_copy_kernel<<<dim3((unsigned)((total + 255) / 256), (unsigned)D.nv, 1), 256, 0, stream()>>>(D, src, B);
     BH_HIP(hipGetLastError());
   }
   return BAMHIP_OK;
