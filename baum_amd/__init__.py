@@ -1036,7 +1036,7 @@ def tune_clear():
 
 def set_rk_fusion(mode=2):
     """RK4 stage update inside the RHS kernels: 0 never, 1 always, 2 on
-    levels of at least 2^22 points (default)"""
+    levels of at least 2^19 points (default)"""
     return lib().bamhip_z4c_set_rk_fusion(int(mode))
 
 

@@ -11,12 +11,13 @@
 #include "../../include/bamhip.h"
 
 /* 0: separate update pass; 1: update fused into the RHS kernels; 2 (default):
- * fused on levels of at least 2^22 points.  The fused store adds the u_c and
- * u_p reads and the u_c / v writes to every RHS block's end: on large levels
- * that costs less than the separate pass, on small levels (a few thousand
- * blocks, where the time goes in rounds of blocks) it costs more. */
+ * fused on levels of at least 2^19 points.  The fused store adds its
+ * operands' reads and writes to every RHS block's end, in place of the
+ * separate passes.  Measured on one MI355X: fused is faster on the 192^3 box
+ * (7.5 M points) and on the levels of the 7-level hierarchy (0.55 to 2.2 M
+ * points), slower on a single 64^3 box (0.29 M points). */
 static int g_fused = 2;
-#define BAMHIP_FUSE_MIN_POINTS (1L << 22)
+#define BAMHIP_FUSE_MIN_POINTS (1L << 19)
 
 int bamhip_z4c_set_rk_fusion(int mode)
 {

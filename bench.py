@@ -68,7 +68,7 @@ def parse():
     ap.add_argument("--rhs-overlap", type=int, default=1,
                     help="1: box-edge shell kernels concurrent with the bulk kernel (second stream)")
     ap.add_argument("--rk-fusion", type=int, default=2,
-                    help="RK4 stage update inside the RHS kernels: 0 never, 1 always, 2 on levels >= 2^22 points")
+                    help="RK4 stage update inside the RHS kernels: 0 never, 1 always, 2 on levels >= 2^19 points")
     ap.add_argument("--xcd-map", type=int, default=1, help="1: XCD-aware tile numbering in the RHS kernels")
     ap.add_argument("--only", default="fmr,box192,c1,c2p,c5")
     return ap.parse_args()

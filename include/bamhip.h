@@ -556,7 +556,7 @@ int bamhip_rk4_last_update(bamhip_level *lev, const int *uc, const int *r, doubl
  * results are bitwise those of the unfused stages when the periodic layers
  * of uc start consistent.
  * bamhip_z4c_set_rk_fusion(mode): 0 never, 1 always, 2 (default) on levels of
- * at least 2^22 points; returns the previous mode. */
+ * at least 2^19 points; returns the previous mode. */
 int bamhip_z4c_rhs_rk4_stage(bamhip_level *lev, const bamhip_z4c_params *par, const int *src,
                              const int *acc, const int *accout, const int *base, const int *vout,
                              const int *copy, double wc, double wv, const int *scratch);
