@@ -163,13 +163,24 @@ struct SkipDst {
   int nd, nv, fill;
   double val;
 };
-__global__ void skip_copy_kernel(SkipDst D, Ptrs25 src, DevBox B) {
+// the boxes of one launch: box b takes blocks [blk0[b], blk0[b + 1])
+constexpr int kSkipBoxes = 16;
+struct SkipBoxes {
+  int n;
+  int blk0[kSkipBoxes + 1];
+  DevBox box[kSkipBoxes];
+};
+static_assert(sizeof(SkipDst) + sizeof(Ptrs25) + sizeof(SkipBoxes) <= 4096, "kernel argument size");
+__global__ void skip_copy_kernel(SkipDst D, Ptrs25 src, SkipBoxes SB) {
+  int b = 0;
+  while (b + 1 < SB.n && (int)blockIdx.x >= SB.blk0[b + 1]) b++;
+  const DevBox B = SB.box[b];
   const int M = B.m, N = B.n, O = B.o;
   const int *s = B.skip;
   const int ix = M - s[0] - s[1], iy = N - s[2] - s[3];
   const long cx = (long)(s[0] + s[1]) * N * O, cy = (long)ix * (s[2] + s[3]) * O,
              cz = (long)ix * iy * (s[4] + s[5]);
-  long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long r = (blockIdx.x - SB.blk0[b]) * (long)blockDim.x + threadIdx.x;
   int i, j, k;
   if (r < cx) {
     const int l = (int)(r % (s[0] + s[1]));
@@ -198,8 +209,21 @@ __global__ void skip_copy_kernel(SkipDst D, Ptrs25 src, DevBox B) {
   for (int t = 0; t < D.nd; t++) D.d[t].p[v][q] = x;
 }
 
-// launch skip_copy_kernel over every box of the level
+// launch skip_copy_kernel over every box of the level, up to kSkipBoxes
+// boxes per launch
 static int skip_launch(bamhip_level *lev, const SkipDst &D, const Ptrs25 &src, const char *what) {
+  SkipBoxes SB{};
+  int blk = 0;
+  auto flush = [&]() -> int {
+    if (SB.n == 0) return BAMHIP_OK;
+    SB.blk0[SB.n] = blk;
+    skip_copy_kernel<<<dim3((unsigned)blk, (unsigned)D.nv, 1), 256, 0, stream()>>>(D, src, SB);
+    BH_HIP(hipGetLastError());
+    SB.n = 0;
+    blk = 0;
+    return BAMHIP_OK;
+  };
+  int e;
   for (const DevBox &B : lev->boxes) {
     const long ix = B.m - B.skip[0] - B.skip[1], iy = B.n - B.skip[2] - B.skip[3];
     if (ix < 1 || iy < 1 || B.o - B.skip[4] - B.skip[5] < 1)
@@ -207,10 +231,12 @@ static int skip_launch(bamhip_level *lev, const SkipDst &D, const Ptrs25 &src, c
     const long total = (long)(B.skip[0] + B.skip[1]) * B.n * B.o + ix * (B.skip[2] + B.skip[3]) * B.o +
                        ix * iy * (B.skip[4] + B.skip[5]);
     if (total <= 0) continue;
-    skip_copy_kernel<<<dim3((unsigned)((total + 255) / 256), (unsigned)D.nv, 1), 256, 0, stream()>>>(D, src, B);
-    BH_HIP(hipGetLastError());
+    if (SB.n == kSkipBoxes && (e = flush())) return e;
+    SB.box[SB.n] = B;
+    SB.blk0[SB.n++] = blk;
+    blk += (int)((total + 255) / 256);
   }
-  return BAMHIP_OK;
+  return flush();
 }
 
 // periodic copy of g layers along direction d for nv variables
