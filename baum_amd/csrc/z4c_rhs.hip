@@ -599,7 +599,11 @@ constexpr int NTHR = 256;
 // PX: unused LDS columns at the end of each tile row (the row stride SX is
 // the staged width WX plus PX), chosen so that the 16 lanes of a read phase
 // fall on distinct banks
-template <int TX_, int TY_, int TZ_, int PX = 0>
+// ORD: which point of the tile thread t of the block computes (64 threads
+// form a wave): 0, x fastest, then y, then z; 1, y fastest, then z, then x
+// (a wave keeps one x, as the layers of an x-face slab do); 2, x fastest,
+// then z, then y (a wave keeps one y).
+template <int TX_, int TY_, int TZ_, int PX = 0, int ORD = 0>
 struct Lay {
   static constexpr int TX = TX_, TY = TY_, TZ = TZ_;
   static constexpr int WX = TX + 2 * HALO;
@@ -607,6 +611,22 @@ struct Lay {
   static constexpr int TILE = SX * SY * SZ;
   static constexpr int NLOAD = (TILE + NTHR - 1) / NTHR;
   static_assert(TX * TY * TZ == NTHR, "one point per thread");
+  static_assert(ORD >= 0 && ORD <= 2, "thread order");
+  __device__ static void coords(int t, int &x, int &y, int &z) {
+    if constexpr (ORD == 1) {
+      y = t % TY;
+      z = (t / TY) % TZ;
+      x = t / (TY * TZ);
+    } else if constexpr (ORD == 2) {
+      x = t % TX;
+      z = (t / TX) % TZ;
+      y = t / (TX * TZ);
+    } else {
+      x = t % TX;
+      y = (t / TX) % TY;
+      z = t / (TX * TY);
+    }
+  }
 };
 using LBulk = Lay<16, 4, 4>;
 // x faces: rows of 2 points, so without padding (stride 8 doubles) the
@@ -981,8 +1001,9 @@ __global__ void __launch_bounds__(NTHR, OCC) z4c_rhs_tile(Z4cPtrs P, TileJobs J,
   G.edge_only = 0;
   const int b = t - J.blk0[r];
   const int ntx = (G.hi[0] - G.lo[0] + L::TX) / L::TX, nty = (G.hi[1] - G.lo[1] + L::TY) / L::TY;
-  const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
-  const int tid = tx + L::TX * (ty + L::TY * tz);
+  const int tid = (int)threadIdx.x;
+  int tx, ty, tz;
+  L::coords(tid, tx, ty, tz);
   const int i0 = G.lo[0] + (b % ntx) * L::TX, j0 = G.lo[1] + ((b / ntx) % nty) * L::TY,
             k0 = G.lo[2] + (b / (ntx * nty)) * L::TZ;
   const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
@@ -1550,7 +1571,7 @@ static int launch_regions(const Z4cPtrs &P, const std::vector<Z4cGeo> &regs, con
     if (blk == 0) continue;
     J.xcd = g_xcd_map;
     const unsigned grid = g_xcd_map ? (unsigned)(8 * ((blk + 7) / 8)) : (unsigned)blk;
-    z4c_rhs_tile<L, SH, NB, VEC, OCC><<<grid, dim3(L::TX, L::TY, L::TZ), 0, st>>>(P, J, K, flag);
+    z4c_rhs_tile<L, SH, NB, VEC, OCC><<<grid, NTHR, 0, st>>>(P, J, K, flag);
     BH_HIP(hipGetLastError());
   }
   return BAMHIP_OK;
@@ -1646,6 +1667,16 @@ static int shell_variant(const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, 
   if ((e = launch_regions<LSlabY, true, NB, VEC, OCC>(P, *R[1], K, flag, st))) return e;
   return launch_regions<LSlabZ, true, NB, VEC, OCC>(P, *R[2], K, flag, st);
 }
+// the slabs with waves that keep one distance to the slab's face
+using LSlabXo = Lay<2, 16, 8, 2, 1>;
+using LSlabYo = Lay<16, 2, 8, 0, 2>;
+template <int NB, int VEC, int OCC>
+static int shell_ord_variant(const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, int *flag, hipStream_t st) {
+  int e;
+  if ((e = launch_regions<LSlabXo, true, NB, VEC, OCC>(P, *R[0], K, flag, st))) return e;
+  if ((e = launch_regions<LSlabYo, true, NB, VEC, OCC>(P, *R[1], K, flag, st))) return e;
+  return launch_regions<LSlabZ, true, NB, VEC, OCC>(P, *R[2], K, flag, st);
+}
 template <int NB, int OCC>
 static int layer_variant(const Z4cPtrs &P, const RegionLists &R, const Z4cK &K, int *flag, hipStream_t st) {
   int e;
@@ -1668,8 +1699,9 @@ static const Variant kVariants[2][kMaxVariants] = {
      {"shell slabs reg", false, false, shell_variant<0, 1, 1>},
      {"shell slabs dma3", true, false, shell_variant<3, 1, 1>},
      {"shell layers dma4", true, false, layer_variant<4, 1>},
-     {"shell slabs dma3 x4", true, true, shell_variant<3, 4, 1>}}};
-static const int kNumVariants[2] = {9, 5};
+     {"shell slabs dma3 x4", true, true, shell_variant<3, 4, 1>},
+     {"shell slabs ord dma3 x4", true, true, shell_ord_variant<3, 4, 1>}}};
+static const int kNumVariants[2] = {9, 6};
 // variant 0 of each kind is the default; variant 1 needs no LDS-DMA
 
 struct TuneEntry {
