@@ -716,16 +716,32 @@ __device__ __forceinline__ void t_hess(TP<L> T, const Z4cGeo &G, double d[6]) {
 struct Sten {
   int o2;               // order-2 stencils (one point from an edge)
   int ko;               // dissipation on (two or more points from every edge)
-  int shift;            // some advection row differs from the standard lop-sided one
   int offp[3], offm[3]; // advection row for beta > 0 / beta < 0 per direction
-  double wx[3];         // beta^a / h_a
+  // advection weights of the point's rows, as the reference forms them
+  // (W = -sign(w) w stencil[offset][k] at offset sign(w) (k - offset),
+  // bam_amr_advection.h:90-146): aw[a][o + 3] at offset o = -3..3 along a
+  // (the centre entries unused), awc the sum of the three centre weights
+  double aw[3][7];
+  double awc;
 };
 
-// neighbour at a run-time offset o along A (|o| <= HALO); T is the thread's
-// tile corner, as for at<>
-template <int A, class L>
-__device__ __forceinline__ double ts(TP<L> T, int o) {
-  return T.p[HALO * (1 + L::SX + L::SX * L::SY) + o * (A == 0 ? 1 : (A == 1 ? L::SX : L::SX * L::SY))];
+// The point's advection weights along A (w = beta^A / h_A), set once per
+// thread: the stages then read the 19 neighbours at fixed LDS offsets.
+template <int A>
+__device__ __forceinline__ void adv_weights(Sten &S, double w) {
+  const int sg = w > 0.0 ? 1 : -1;
+  const int off = w > 0.0 ? S.offp[A] : S.offm[A];
+#pragma unroll
+  for (int o = -3; o <= 3; o++) {
+    const int k = off + sg * o;  // row entry k lands on offset sg (k - off) = o
+    double c = 0.0;
+    if (S.o2) {
+      if (k >= 0 && k < 3) c = kAdvSt2[off][k];
+    } else {
+      if (k >= 0 && k < 5) c = kAdvSt4[off][k];
+    }
+    S.aw[A][o + 3] = (w == 0.0) ? 0.0 : -sg * w * c;
+  }
 }
 
 template <bool SH, int A, class L>
@@ -759,26 +775,16 @@ __device__ __forceinline__ void s_hess(TP<L> T, const Z4cGeo &G, const Sten &S, 
   }
   t_hess(T, G, d);
 }
-// advection along A with the reference's row selection (Edge::set_adv)
+// advection with the point's weights (shell) or the lop-sided rows (bulk)
 template <int A, class L>
-__device__ __forceinline__ double s_adv1(TP<L> T, const Sten &S) {
-  const double wx = S.wx[A];
-  if (wx == 0.0) return 0.0;
-  const int sg = wx > 0.0 ? 1 : -1;
-  const int off = wx > 0.0 ? S.offp[A] : S.offm[A];
-  double acc = 0.0;
-  if (S.o2) {
-#pragma unroll
-    for (int k = 0; k < 3; k++) acc += kAdvSt2[off][k] * ts<A>(T, sg * (k - off));
-  } else {
-#pragma unroll
-    for (int k = 0; k < 5; k++) acc += kAdvSt4[off][k] * ts<A>(T, sg * (k - off));
-  }
-  return -sg * wx * acc;
+__device__ __forceinline__ double s_advw1(TP<L> T, const double w[7]) {
+  return w[0] * ax<A, -3>(T) + w[1] * ax<A, -2>(T) + w[2] * ax<A, -1>(T) + w[4] * ax<A, 1>(T) +
+         w[5] * ax<A, 2>(T) + w[6] * ax<A, 3>(T);
 }
 template <bool SH, class L>
 __device__ __forceinline__ double s_adv(TP<L> T, const double pw[3], const double mw[3], const Sten &S) {
-  if (SH && S.shift) return s_adv1<0>(T, S) + s_adv1<1>(T, S) + s_adv1<2>(T, S);
+  if constexpr (SH)
+    return S.awc * at<0, 0, 0>(T) + s_advw1<0>(T, S.aw[0]) + s_advw1<1>(T, S.aw[1]) + s_advw1<2>(T, S.aw[2]);
   return t_adv(T, pw, mw);
 }
 template <bool SH, class L>
@@ -837,9 +843,10 @@ struct Stager {
 // chunks of 64 units, NPER per wave.  VEC = 1: a unit is one dword, the low
 // (even unit) or high half of a tile element; VEC = 4: a unit is 16 bytes,
 // the two tile elements (x, x + 1) with x even, which needs an even row
-// length and 16-byte aligned pairs in HBM (checked by the host).  Slots
-// outside the box load the box's first point (finite data that no computed
-// point reads).
+// length and 16-byte aligned pairs in HBM (checked by the host).  Slots of
+// the tile outside the box load the nearest point of the box (a pair stays a
+// pair), so that every staged value is box data: the shell's advection reads
+// such slots with weight 0.
 #define BH_LDS(p) ((__attribute__((address_space(3))) void *)(p))
 template <class L, int NB, int VEC>
 struct DmaStager {
@@ -857,15 +864,18 @@ struct DmaStager {
   __device__ DmaStager(const Z4cGeo &G, int i0, int j0, int k0, int tid) {
     const int lane = tid & 63;
     wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const long cbase = G.noffset + (i0 - HALO) + G.sj * (j0 - HALO) + G.sk * (k0 - HALO);
 #pragma unroll
     for (int c = 0; c < NPER; c++) {
       const int d = (wave * NPER + c) * 64 + lane;
       const int e = VEC == 1 ? d >> 1 : 2 * d;
       const int x = e % L::SX, y = (e / L::SX) % L::SY, z = e / (L::SX * L::SY);
       const int gi = i0 - HALO + x, gj = j0 - HALO + y, gk = k0 - HALO + z;
-      const bool in = e < L::TILE && x < L::WX && gi >= 0 && gj >= 0 && gk >= 0 && gi < G.m && gj < G.n && gk < G.o;
-      const long el = in ? cbase + x + G.sj * y + G.sk * z : G.noffset;
+      const bool in = e < L::TILE && x < L::WX;
+      // clamped into the box; for 16-byte units gi is even and so is the
+      // box's row length, so the clamped pair is (0, 1) or (m - 2, m - 1)
+      const int ci = min(max(gi, 0), G.m - (VEC == 1 ? 1 : 2)), cj = min(max(gj, 0), G.n - 1),
+                ck = min(max(gk, 0), G.o - 1);
+      const long el = in ? G.noffset + ci + G.sj * cj + G.sk * ck : G.noffset;
       boff[c] = (unsigned)(el * 8 + (VEC == 1 && in ? (d & 1) * 4 : 0));
     }
   }
@@ -1020,7 +1030,6 @@ __global__ void __launch_bounds__(NTHR, OCC) z4c_rhs_tile(Z4cPtrs P, TileJobs J,
     active = active && dmin <= 2;
     S.o2 = dmin == 1;
     S.ko = dmin >= 2;
-    S.shift = S.o2;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       const int dl = pos[a], dh = pmax[a] - pos[a];
@@ -1030,7 +1039,6 @@ __global__ void __launch_bounds__(NTHR, OCC) z4c_rhs_tile(Z4cPtrs P, TileJobs J,
       } else {
         S.offm[a] = (dl >= 3 ? 0 : 3 - dl) + 1;
         S.offp[a] = (dh >= 3 ? 0 : 3 - dh) + 1;
-        S.shift |= (S.offm[a] != 1 || S.offp[a] != 1);
       }
     }
   }
@@ -1100,7 +1108,12 @@ __global__ void __launch_bounds__(NTHR, OCC) z4c_rhs_tile(Z4cPtrs P, TileJobs J,
     const double w = beta[a] * G.ih[a];
     pw[a] = (w > 0.0 ? w : 0.0) * (1.0 / 12.0);
     mw[a] = (w < 0.0 ? w : 0.0) * (1.0 / 12.0);
-    if (SH) S.wx[a] = w;
+  }
+  if constexpr (SH) {
+    adv_weights<0>(S, beta[0] * G.ih[0]);
+    adv_weights<1>(S, beta[1] * G.ih[1]);
+    adv_weights<2>(S, beta[2] * G.ih[2]);
+    S.awc = S.aw[0][3] + S.aw[1][3] + S.aw[2][3];
   }
   const bool ko = (K.order_dissipation == 4);
   const double eps = ko ? K.dissfactor : 0.0;
