@@ -149,6 +149,28 @@ __global__ void __launch_bounds__(kBlock) prolong_kernel(PArgs a, VarPtrs vp) {
     }
 }
 
+// Stage the SX x SY x SZ box at src (row strides sj, sk) into the LDS array
+// t, 0 outside [0, ex) x [0, ey) x [0, ez): every thread's loads are issued
+// before its first LDS store, so a block waits for one memory round trip,
+// not one per element it stages.
+template <int SX, int SY, int SZ>
+__device__ __forceinline__ void stage_tile(double *t, const double *__restrict__ src, long sj, long sk, int ex, int ey,
+                                           int ez, int tid) {
+  constexpr int NT = SX * SY * SZ, NL = (NT + kBlock - 1) / kBlock;
+  double r[NL];
+#pragma unroll
+  for (int l = 0; l < NL; l++) {
+    const int idx = tid + l * kBlock;
+    const int x = idx % SX, y = (idx / SX) % SY, z = idx / (SX * SY);
+    r[l] = (idx < NT && x < ex && y < ey && z < ez) ? src[x + (long)y * sj + (long)z * sk] : 0.0;
+  }
+#pragma unroll
+  for (int l = 0; l < NL; l++) {
+    const int idx = tid + l * kBlock;
+    if (idx < NT) t[idx] = r[l];
+  }
+}
+
 // Prolongation of a tile of PTX x PTY x PTZ octets through LDS: the coarse
 // footprint is staged once, then the z sums of every coarse (x, y) column for
 // both children in z, the y sums for both children in y and the x sums, as
@@ -189,10 +211,7 @@ __global__ void __launch_bounds__(kBlock) prolong_tile_kernel(PArgs a, VarPtrs v
   const double *__restrict__ cv =
       vp.src[v] + J.coff + (J.icb[0] + ox0) + (long)(J.icb[2] + oy0) * J.cm + (long)(J.icb[4] + oz0) * J.cmn;
   const int ex = vx + N, ey = vy + N, ez = vz + N;  // coarse points the valid octets read
-  for (int idx = tid; idx < CZ * CY * CX; idx += kBlock) {
-    const int x = idx % CX, y = (idx / CX) % CY, z = idx / (CX * CY);
-    cin[idx] = (x < ex && y < ey && z < ez) ? cv[x + (long)y * J.cm + (long)z * J.cmn] : 0.0;
-  }
+  stage_tile<CX, CY, CZ>(cin, cv, J.cm, J.cmn, ex, ey, ez, tid);
   __syncthreads();
   for (int idx = tid; idx < 2 * PTZ * CY * CX; idx += kBlock) {  // z sums
     const int x = idx % CX, y = (idx / CX) % CY, fz = idx / (CX * CY);
@@ -303,10 +322,7 @@ __global__ void __launch_bounds__(kBlock) restrict_tile_kernel(RArgs a, VarPtrs 
   const int v = blockIdx.y;
   const double *__restrict__ src = vp.src[v] + J.foff + (J.ifb[0] + 2 * tx0) + (long)(J.ifb[2] + 2 * ty0) * J.fm +
                                    (long)(J.ifb[4] + 2 * tz0) * J.fmn;
-  for (int idx = tid; idx < FX * FY * FZ; idx += kBlock) {
-    const int x = idx % FX, y = (idx / FX) % FY, z = idx / (FX * FY);
-    fin[idx] = (x < ex && y < eyy && z < ezz) ? src[x + (long)y * J.fm + (long)z * J.fmn] : 0.0;
-  }
+  stage_tile<FX, FY, FZ>(fin, src, J.fm, J.fmn, ex, eyy, ezz, tid);
   __syncthreads();
   for (int idx = tid; idx < RTZ * FY * FX; idx += kBlock) {  // z sums
     const int x = idx % FX, y = (idx / FX) % FY, cz = idx / (FX * FY);
