@@ -32,7 +32,11 @@ Workloads (synthetic data of the configurations' shapes, FP64 throughout):
 
 `roofline` is for the dominant kernel, z4c_rhs_bulk: achieved = 5.5e3
 algorithmic FP64 FLOP per point x points per launch / the launch's average
-duration from HIP events on the library stream.  `cpu_baseline` times the
+duration from HIP events on the library stream.  `fmr.rhs_fp64` gives the
+same rate for the headline workload: 5.5e3 FLOP x 4 RHS evaluations per point
+update x the updates of one coarse step / the time of all its RHS launches
+(bulk and shell tiles, RK4 update fused into the store) in the eager split.
+`cpu_baseline` times the
 OpenMP CPU oracle (the restatement used as the parity checker) on a bounded
 sample of the fmr workload's per-box work on the host cores.
 """
@@ -768,6 +772,15 @@ def main():
                          "updates_per_coarse_step": fmr["updates_per_step"], "boxes": fmr["boxes"],
                          "parallelism": "single GPU"}
         out["fmr"] = {k: fmr[k] for k in ("nonfinite", "eager_split_ms", "punctures_end")}
+        rhs_ms = fmr["eager_split_ms"]["z4_rhs"]
+        rhs_flop = FLOP_PER_POINT_RHS * 4 * fmr["updates_per_step"]
+        if rhs_ms > 0:
+            rate = rhs_flop / (rhs_ms * 1e-3) / 1e12
+            out["fmr"]["rhs_fp64"] = {"achieved": rate, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                                      "frac": rate / PEAK_FP64_TFLOPS, "rhs_ms_per_coarse_step": rhs_ms,
+                                      "flop_per_coarse_step": rhs_flop,
+                                      "kernels": "every Z4c RHS launch of one eagerly run coarse step: bulk and "
+                                                 "shell tiles on two streams, RK4 update fused into the store"}
     elif box:
         out.update(value=box["value"], unit="grid-point-updates/s", ms_per_step=box["ms_per_step"])
         out["config"] = {"workload": "Z4c RK4, one 192^3 periodic box", "parallelism": "single GPU"}
