@@ -723,6 +723,11 @@ struct Sten {
   // (the centre entries unused), awc the sum of the three centre weights
   double aw[3][7];
   double awc;
+  // first derivative along a: c1[a] (f(+1) - f(-1)) + c2[a] (f(-2) - f(+2)),
+  // 1/(2h), 0 at order 2 and 8/(12h), 1/(12h) at order 4; kof: 1 where
+  // dissipation is on, else 0
+  double c1[3], c2[3];
+  double kof;
 };
 
 // The point's advection weights along A (w = beta^A / h_A), set once per
@@ -746,7 +751,7 @@ __device__ __forceinline__ void adv_weights(Sten &S, double w) {
 
 template <bool SH, int A, class L>
 __device__ __forceinline__ double s_d1(TP<L> T, const Z4cGeo &G, const Sten &S) {
-  if (SH && S.o2) return (ax<A, 1>(T) - ax<A, -1>(T)) * G.i2h[A];
+  if constexpr (SH) return S.c2[A] * (ax<A, -2>(T) - ax<A, 2>(T)) + S.c1[A] * (ax<A, 1>(T) - ax<A, -1>(T));
   return t_d1<A>(T, G);
 }
 template <bool SH, class L>
@@ -789,7 +794,7 @@ __device__ __forceinline__ double s_adv(TP<L> T, const double pw[3], const doubl
 }
 template <bool SH, class L>
 __device__ __forceinline__ double s_ko(TP<L> T, const Z4cGeo &G, const Sten &S) {
-  if (SH && !S.ko) return 0.0;
+  if constexpr (SH) return S.kof * t_ko(T, G);
   return t_ko(T, G);
 }
 
@@ -1114,6 +1119,12 @@ __global__ void __launch_bounds__(NTHR, OCC) z4c_rhs_tile(Z4cPtrs P, TileJobs J,
     adv_weights<1>(S, beta[1] * G.ih[1]);
     adv_weights<2>(S, beta[2] * G.ih[2]);
     S.awc = S.aw[0][3] + S.aw[1][3] + S.aw[2][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      S.c1[a] = S.o2 ? G.i2h[a] : 8.0 * G.i12h[a];
+      S.c2[a] = S.o2 ? 0.0 : G.i12h[a];
+    }
+    S.kof = S.ko ? 1.0 : 0.0;
   }
   const bool ko = (K.order_dissipation == 4);
   const double eps = ko ? K.dissfactor : 0.0;
