@@ -711,8 +711,13 @@ __device__ __forceinline__ void t_hess(TP<L> T, const Z4cGeo &G, double d[6]) {
 // centred differences and order-2 advection rows, dissipation needs two or
 // more points to every edge, and the advection row shifts away from an edge
 // closer than rstencil.  The shell instantiation (SH = true) of the tile
-// kernel selects these per point; every selected stencil stays inside the
-// box, hence inside the tile's 3-point halo.
+// kernel selects these per point.  Every stencil entry with a nonzero
+// coefficient lies inside the box, hence inside the tile's 3-point halo.
+// The advection weights, first-derivative and dissipation coefficients are
+// set once per point, so every point reads the same neighbours at fixed LDS
+// offsets; a neighbour outside the box is read with coefficient 0 from a
+// staging slot that holds box data.  Only the second derivatives branch on
+// the point's order.
 struct Sten {
   int o2;               // order-2 stencils (one point from an edge)
   int ko;               // dissipation on (two or more points from every edge)
