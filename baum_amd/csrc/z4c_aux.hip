@@ -1,6 +1,7 @@
 // z4c_aux.hip -- the streaming and boundary kernels around the Z4c RHS:
 // SoA vector ops and the fused RK4 stage update, the radiative and periodic
 // boundaries, the algebraic constraints and the Hamiltonian-constraint norm.
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -178,27 +179,27 @@ __global__ void skip_copy_kernel(SkipDst D, Ptrs25 src, SkipBoxes SB) {
   const int M = B.m, N = B.n, O = B.o;
   const int *s = B.skip;
   const int ix = M - s[0] - s[1], iy = N - s[2] - s[3];
-  const long cx = (long)(s[0] + s[1]) * N * O, cy = (long)ix * (s[2] + s[3]) * O,
-             cz = (long)ix * iy * (s[4] + s[5]);
-  long r = (blockIdx.x - SB.blk0[b]) * (long)blockDim.x + threadIdx.x;
+  // 32-bit index arithmetic: the host keeps a box's skip points below 2^31
+  const int cx = (s[0] + s[1]) * N * O, cy = ix * (s[2] + s[3]) * O, cz = ix * iy * (s[4] + s[5]);
+  int r = ((int)blockIdx.x - SB.blk0[b]) * (int)blockDim.x + (int)threadIdx.x;
   int i, j, k;
   if (r < cx) {
-    const int l = (int)(r % (s[0] + s[1]));
+    const int l = r % (s[0] + s[1]);
     r /= (s[0] + s[1]);
     i = l < s[0] ? l : M - s[1] + (l - s[0]);
-    j = (int)(r % N);
-    k = (int)(r / N);
+    j = r % N;
+    k = r / N;
   } else if ((r -= cx) < cy) {
-    i = s[0] + (int)(r % ix);
+    i = s[0] + r % ix;
     r /= ix;
-    const int l = (int)(r % (s[2] + s[3]));
+    const int l = r % (s[2] + s[3]);
     j = l < s[2] ? l : N - s[3] + (l - s[2]);
-    k = (int)(r / (s[2] + s[3]));
+    k = r / (s[2] + s[3]);
   } else if ((r -= cy) < cz) {
-    i = s[0] + (int)(r % ix);
+    i = s[0] + r % ix;
     r /= ix;
-    j = s[2] + (int)(r % iy);
-    const int l = (int)(r / iy);
+    j = s[2] + r % iy;
+    const int l = r / iy;
     k = l < s[4] ? l : O - s[5] + (l - s[4]);
   } else {
     return;
@@ -231,6 +232,8 @@ static int skip_launch(bamhip_level *lev, const SkipDst &D, const Ptrs25 &src, c
     const long total = (long)(B.skip[0] + B.skip[1]) * B.n * B.o + ix * (B.skip[2] + B.skip[3]) * B.o +
                        ix * iy * (B.skip[4] + B.skip[5]);
     if (total <= 0) continue;
+    if (total > (long)INT_MAX - 255)
+      return set_error(BAMHIP_EINVAL, "%s: a box has %ld skip points (at most 2^31 - 256)", what, total);
     if (SB.n == kSkipBoxes && (e = flush())) return e;
     SB.box[SB.n] = B;
     SB.blk0[SB.n++] = blk;
