@@ -159,6 +159,34 @@ def test_rhs_tile_variants_bitwise_equal_on_uneven_level():
     lev.free()
 
 
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_shell_rows_with_radial_shift_match_oracle(sign):
+    """A shift pointing out of the box (sign +1) or into it (-1) on every face:
+    the box-edge shell then takes the shifted advection rows on every face,
+    edge and corner (order-4 rows two points from an edge, order-2 rows one
+    point from it), together with the order-2 derivatives and the dissipation
+    switched off one point from an edge.  The shell's per-point advection
+    weights and derivative and dissipation coefficients must reproduce the
+    oracle's per-point rules to 1e-12 of each field's magnitude."""
+    h = 0.2
+    ob = O.make_box(24, 22, 20, -2.3, -2.1, -1.9, h, skip=(1,) * 6)
+    X = grid(ob)
+    u = puncture_z4c(X, masses=((1.0, (0.05, -0.02, 0.03)),))
+    rng = np.random.default_rng(29)
+    for a in range(3):
+        c = 0.5 * (X[a].min() + X[a].max())
+        half = 0.5 * (X[a].max() - X[a].min())
+        u[19 + a] = sign * 0.4 * (X[a] - c) / half + 0.02 * rng.standard_normal(X.shape[1:])
+    u[13] = 0.05 * np.sin(X[0]) * np.cos(X[1])
+    cpu, _ = O.z4c_rhs(ob, O.z4c_params(), u)
+    lev = _level_for(ob, (B.PHYBOUND,) * 6, nvars=50)
+    lev.put_state(U, u.reshape(25, -1))
+    lev.z4c_rhs(B.z4c_params(), R, U)
+    err = _rel_err(lev.get_state(R).reshape(u.shape), cpu)
+    assert err.max() <= 1e-12, dict(zip(B.VAR_NAMES, err))
+    lev.free()
+
+
 def test_rhs_pair_variants_rejected_on_odd_rows():
     """16-byte staging needs even box rows: on a 21-wide box a pinned pair
     variant is refused, and the tuner skips it and still matches the oracle"""
