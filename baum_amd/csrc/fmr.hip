@@ -195,18 +195,19 @@ __global__ void __launch_bounds__(kBlock) prolong_tile_kernel(PArgs a, VarPtrs v
   const int vx = min(PTX, J.nox - ox0), vy = min(PTY, J.noy - oy0), vz = min(PTZ, J.noz - oz0);
   const int tid = threadIdx.x;
   // does any fine point of the tile get a value?  (fbox0 skip, :845-850)
-  int mine = 0;
-  for (int idx = tid; idx < 8 * PTX * PTY * PTZ; idx += kBlock) {
-    const int fx = idx % (2 * PTX), fy = (idx / (2 * PTX)) % (2 * PTY), fz = idx / (4 * PTX * PTY);
-    const int ox = fx / 2, oy = fy / 2, oz = fz / 2;
-    if (ox >= vx || oy >= vy || oz >= vz) continue;
-    const int fi = J.ifb[0] + 2 * (ox0 + ox), j = J.ifb[2] + 2 * oy0 + fy, k = J.ifb[4] + 2 * oz0 + fz;
-    const bool valid = j <= J.ifb[3] && k <= J.ifb[5];
-    const bool skip = fi >= J.ifb0[0] && fi <= J.ifb0[1] && j >= J.ifb0[2] && j <= J.ifb0[3] && k >= J.ifb0[4] &&
-                      k <= J.ifb0[5];
-    mine |= (valid && !skip);
+  // The tile's valid points (j <= ifb[3], k <= ifb[5]) form a box, with the
+  // x test on each octet's left child fi, and so does the skipped set, so the
+  // tile has nothing to set exactly when its valid box is empty or lies in
+  // the skipped box: a test on block-uniform values, before any barrier.
+  {
+    const int xlo = J.ifb[0] + 2 * ox0, xhi = J.ifb[0] + 2 * (ox0 + vx - 1);
+    const int ylo = J.ifb[2] + 2 * oy0, yhi = min(J.ifb[2] + 2 * (oy0 + vy) - 1, J.ifb[3]);
+    const int zlo = J.ifb[4] + 2 * oz0, zhi = min(J.ifb[4] + 2 * (oz0 + vz) - 1, J.ifb[5]);
+    const bool none = yhi < ylo || zhi < zlo;
+    const bool inside = xlo >= J.ifb0[0] && xhi <= J.ifb0[1] && ylo >= J.ifb0[2] && yhi <= J.ifb0[3] &&
+                        zlo >= J.ifb0[4] && zhi <= J.ifb0[5];
+    if (none || inside) return;
   }
-  if (!__syncthreads_or(mine)) return;
   const int v = blockIdx.y;
   const double *__restrict__ cv =
       vp.src[v] + J.coff + (J.icb[0] + ox0) + (long)(J.icb[2] + oy0) * J.cm + (long)(J.icb[4] + oz0) * J.cmn;
